@@ -1,0 +1,160 @@
+/*
+ * OneFFT for AMD Instinct MI355X (gfx950): power-of-two, complex-to-complex, single-precision,
+ * out-of-place FFT on the GPU behind the OneFFT C API.
+ *
+ * Drop-in: the first part of this header declares the same functions, types, macros and C++
+ * wrappers as OneFFT's include/onefft.h (reference include/onefft.h:53-170), with the same
+ * meaning, so code written against OneFFT compiles and runs unchanged:
+ *
+ *     forward:  out[k] = sum over j of in[j] * exp(-2 pi i j k / n)
+ *     inverse:  out[k] = sum over j of in[j] * exp(+2 pi i j k / n)
+ *
+ * Data are n complex values stored as 2n interleaved floats (re, im, ...), the layout of C's
+ * `float complex` and C++'s `std::complex<float>`. Nothing is normalised: forward then inverse
+ * gives n times the input. n is a power of two from 1 to 2^30.
+ *
+ * On this implementation the transform runs on the GPU. The drop-in functions accept host
+ * pointers (copied to and from the device around the transform, which then dominates the cost)
+ * and device pointers (hipMalloc'd memory, transformed in place on the device); they return when
+ * `out` holds the result. The second part of the header adds asynchronous device-pointer,
+ * batched and ragged entry points on a caller-supplied HIP stream.
+ */
+#ifndef ONEFFT_H
+#define ONEFFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#define ONEFFT_VERSION_MAJOR 0
+#define ONEFFT_VERSION_MINOR 1
+#define ONEFFT_VERSION_PATCH 0
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ==== drop-in API (reference include/onefft.h:61-116) ==================================== */
+
+/* Transform n complex values from `in` to `out` (reference include/onefft.h:61-77).
+ * The buffers must not overlap; no alignment is required. The first call for a size creates its
+ * plan (device tables) and keeps it until onefft_cache_clear(); later calls find it with one
+ * atomic load. Returns 0, or -1 leaving `out` untouched if n is not a power of two from 1 to
+ * 2^30 or the plan could not be created. */
+int onefft_forward(const float *in, float *out, size_t n);
+int onefft_inverse(const float *in, float *out, size_t n);
+
+/* Creates the plan for size n now (reference include/onefft.h:79-82), and sizes the calling
+ * thread's device staging buffers for n, so later calls of that size on this thread allocate
+ * nothing. Returns 0, or -1 if n is invalid or on allocation failure. */
+int onefft_cache_prepare(size_t n);
+
+/* Frees every plan created by the functions above (reference include/onefft.h:84-86). Must not
+ * run while another thread is inside one of them. */
+void onefft_cache_clear(void);
+
+/* A plan holds the device tables for one size and serves both directions
+ * (reference include/onefft.h:88-109). */
+typedef struct onefft_plan onefft_plan;
+
+/* NULL if n is not a power of two from 1 to 2^30 or if the device allocation fails. The plan
+ * belongs to the HIP device that is current when it is created. */
+onefft_plan *onefft_plan_create(size_t n);
+
+/* Frees a plan; NULL is allowed. */
+void onefft_plan_destroy(onefft_plan *plan);
+
+/* Transform from `in` to `out` (host or device pointers, no alignment required, no overlap).
+ * The plan is never modified: any number of threads may use one plan at once, in either
+ * direction. The first call on a thread creates that thread's stream and staging buffers. A HIP
+ * failure leaves a message for onefft_last_error(). */
+void onefft_plan_forward(const onefft_plan *plan, const float *in, float *out);
+void onefft_plan_inverse(const onefft_plan *plan, const float *in, float *out);
+
+/* The number of complex values a plan transforms. */
+size_t onefft_plan_size(const onefft_plan *plan);
+
+/* The kernels in use: "hip-gfx950". */
+const char *onefft_implementation(void);
+
+/* ==== GPU extensions (no counterpart in the reference) ===================================== */
+
+/* hipStream_t without including the HIP headers: the same type. NULL is the default stream. */
+struct ihipStream_t;
+
+/* The last error of a call on this thread that failed on the GPU side, or "" if none. */
+const char *onefft_last_error(void);
+
+/* Device-pointer transform, asynchronous on `stream` (reference counterpart:
+ * onefft_plan_forward/inverse, include/onefft.h:105-106, which run synchronously on the host).
+ * d_in and d_out are device memory, 8-byte aligned, not overlapping; d_in is not modified.
+ * Returns 0, or -1 (invalid arguments, launch failure; see onefft_last_error()). */
+int onefft_plan_forward_device(const onefft_plan *plan, const float *d_in, float *d_out, struct ihipStream_t *stream);
+int onefft_plan_inverse_device(const onefft_plan *plan, const float *d_in, float *d_out, struct ihipStream_t *stream);
+
+/* `batch` transforms of plan's size in one launch sequence (no reference counterpart; the
+ * reference transforms one array per call). Transform b reads d_in + 2*b*in_dist floats and
+ * writes d_out + 2*b*out_dist floats; distances are in complex values, at least the size, and the
+ * output regions must not overlap each other or the input. Asynchronous on `stream`. Returns 0
+ * or -1. */
+int onefft_plan_forward_batch(const onefft_plan *plan, const float *d_in, float *d_out, size_t batch,
+                              size_t in_dist, size_t out_dist, struct ihipStream_t *stream);
+int onefft_plan_inverse_batch(const onefft_plan *plan, const float *d_in, float *d_out, size_t batch,
+                              size_t in_dist, size_t out_dist, struct ihipStream_t *stream);
+
+#ifdef __cplusplus
+} /* extern "C" */
+
+#include <complex>
+
+/* C++ interface (reference include/onefft.h:121-170): the same names and behaviour. */
+namespace onefft {
+
+/* False if n is not a power of two from 1 to 2^30 or the plan could not be created. */
+inline bool forward(const std::complex<float> *in, std::complex<float> *out, std::size_t n) {
+    return onefft_forward(reinterpret_cast<const float *>(in), reinterpret_cast<float *>(out), n) == 0;
+}
+inline bool inverse(const std::complex<float> *in, std::complex<float> *out, std::size_t n) {
+    return onefft_inverse(reinterpret_cast<const float *>(in), reinterpret_cast<float *>(out), n) == 0;
+}
+
+/* Owns a plan; move-only. */
+class Plan {
+public:
+    explicit Plan(std::size_t n) : plan_(onefft_plan_create(n)) {}
+    ~Plan() { onefft_plan_destroy(plan_); }
+    Plan(Plan &&other) noexcept : plan_(other.plan_) { other.plan_ = nullptr; }
+    Plan &operator=(Plan &&other) noexcept {
+        if (this != &other) {
+            onefft_plan_destroy(plan_);
+            plan_ = other.plan_;
+            other.plan_ = nullptr;
+        }
+        return *this;
+    }
+    Plan(const Plan &) = delete;
+    Plan &operator=(const Plan &) = delete;
+
+    /* False if the size was invalid or the plan could not be created; such a plan must not be used. */
+    explicit operator bool() const { return plan_ != nullptr; }
+    std::size_t size() const { return onefft_plan_size(plan_); }
+
+    void forward(const std::complex<float> *in, std::complex<float> *out) const {
+        onefft_plan_forward(plan_, reinterpret_cast<const float *>(in), reinterpret_cast<float *>(out));
+    }
+    void inverse(const std::complex<float> *in, std::complex<float> *out) const {
+        onefft_plan_inverse(plan_, reinterpret_cast<const float *>(in), reinterpret_cast<float *>(out));
+    }
+    void forward(const float *in, float *out) const { onefft_plan_forward(plan_, in, out); }
+    void inverse(const float *in, float *out) const { onefft_plan_inverse(plan_, in, out); }
+
+    /* The underlying C plan, for the GPU extensions above. */
+    const onefft_plan *get() const { return plan_; }
+
+private:
+    onefft_plan *plan_;
+};
+
+} /* namespace onefft */
+#endif /* __cplusplus */
+
+#endif /* ONEFFT_H */

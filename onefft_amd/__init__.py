@@ -1,0 +1,188 @@
+"""Python view of the OneFFT gfx950 library (``libonefft_amd.so``) through its C ABI.
+
+The product is the C library declared in ``include/onefft.h``; this module is the thin binding
+the tests and the benchmark use:
+
+* ``lib()`` — the ctypes handle with argument types set for every exported function;
+* ``Plan`` — owns an ``onefft_plan`` (reference ``include/onefft.h:88-109``);
+* ``fft(x, inverse=False)`` — transforms a complex64 CUDA tensor along its last dimension through
+  the batched device entry point on torch's current stream;
+* ``FFT`` — a ``torch.autograd.Function`` whose backward is the adjoint: the adjoint of the
+  unnormalised forward DFT is the unnormalised inverse (and vice versa), run by the same kernels
+  and tables.
+
+There is no fallback: if the library is missing, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_float, c_int, c_size_t, c_void_p
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libonefft_amd.so")
+
+# Every function include/onefft.h declares, with its C signature.
+_F = POINTER(c_float)
+SIGNATURES = {
+    "onefft_forward": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "onefft_inverse": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "onefft_cache_prepare": (c_int, [c_size_t]),
+    "onefft_cache_clear": (None, []),
+    "onefft_plan_create": (c_void_p, [c_size_t]),
+    "onefft_plan_destroy": (None, [c_void_p]),
+    "onefft_plan_forward": (None, [c_void_p, c_void_p, c_void_p]),
+    "onefft_plan_inverse": (None, [c_void_p, c_void_p, c_void_p]),
+    "onefft_plan_size": (c_size_t, [c_void_p]),
+    "onefft_implementation": (c_char_p, []),
+    "onefft_last_error": (c_char_p, []),
+    "onefft_plan_forward_device": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "onefft_plan_inverse_device": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "onefft_plan_forward_batch": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t, c_size_t, c_size_t, c_void_p]),
+    "onefft_plan_inverse_batch": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t, c_size_t, c_size_t, c_void_p]),
+}
+
+_lib = None
+
+
+class OneFFTError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded library. Raises if it has not been built (``python -m onefft_amd.build``)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OneFFTError(f"{LIB_PATH} is missing: build it with `python -m onefft_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().onefft_last_error().decode()
+
+
+def implementation() -> str:
+    return lib().onefft_implementation().decode()
+
+
+def _check(status: int, what: str) -> None:
+    if status != 0:
+        raise OneFFTError(f"{what} failed: {last_error()}")
+
+
+class Plan:
+    """An onefft_plan for n points (both directions)."""
+
+    def __init__(self, n: int):
+        self.n = int(n)
+        self._p = lib().onefft_plan_create(self.n)
+        if not self._p:
+            raise OneFFTError(f"onefft_plan_create({n}) returned NULL: {last_error()}")
+
+    @property
+    def handle(self) -> int:
+        return self._p
+
+    def close(self) -> None:
+        if self._p:
+            lib().onefft_plan_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # synchronous drop-in calls: host (numpy) or device pointers
+    def forward_ptr(self, src: int, dst: int) -> None:
+        lib().onefft_plan_forward(self._p, src, dst)
+
+    def inverse_ptr(self, src: int, dst: int) -> None:
+        lib().onefft_plan_inverse(self._p, src, dst)
+
+    # asynchronous device calls
+    def batch_ptr(self, src: int, dst: int, batch: int, in_dist: int, out_dist: int, inverse: bool,
+                  stream: int = 0) -> None:
+        fn = lib().onefft_plan_inverse_batch if inverse else lib().onefft_plan_forward_batch
+        _check(fn(self._p, src, dst, batch, in_dist, out_dist, stream), "onefft_plan_*_batch")
+
+
+_plans: dict[int, Plan] = {}
+
+
+def plan(n: int) -> Plan:
+    """A process-wide plan per size (created on first use, like the library's own cache)."""
+    p = _plans.get(n)
+    if p is None:
+        p = _plans[n] = Plan(n)
+    return p
+
+
+def _stream_ptr(torch, device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def fft(x, inverse: bool = False, out=None):
+    """Unnormalised DFT of a complex64 CUDA tensor along its last dimension (power-of-two length),
+    asynchronous on torch's current stream."""
+    import torch
+
+    if x.dtype != torch.complex64 or not x.is_cuda:
+        raise OneFFTError("fft expects a complex64 CUDA tensor")
+    x = x.contiguous()
+    n = x.shape[-1]
+    batch = x.numel() // n if n else 0
+    if out is None:
+        out = torch.empty_like(x)
+    if batch == 0:
+        return out
+    plan(n).batch_ptr(x.data_ptr(), out.data_ptr(), batch, n, n, inverse, _stream_ptr(torch, x.device))
+    return out
+
+
+def _autograd():
+    import torch
+
+    class FFT(torch.autograd.Function):
+        """y = F x (forward) or F^H x (inverse), unnormalised, along the last dimension.
+
+        Backward applies the adjoint: for y = F x, dL/dx = F^H dL/dy (PyTorch's conjugate
+        Wirtinger convention), and F^H is the unnormalised inverse transform, so the backward of
+        one direction is the other direction on the same plan.
+        """
+
+        @staticmethod
+        def forward(ctx, x, inverse: bool = False):
+            ctx.inverse = bool(inverse)
+            return fft(x, inverse=bool(inverse))
+
+        @staticmethod
+        def backward(ctx, g):
+            return fft(g.contiguous(), inverse=not ctx.inverse), None
+
+    return FFT
+
+
+_FFT = None
+
+
+def fft_autograd(x, inverse: bool = False):
+    """Differentiable unnormalised FFT (see FFT)."""
+    global _FFT
+    if _FFT is None:
+        _FFT = _autograd()
+    return _FFT.apply(x, inverse)

@@ -1,0 +1,206 @@
+// Device building blocks shared by every OneFFT gfx950 kernel: the complex type, constant
+// twiddles, register DFTs of radix 2..16, the Stockham stage schedule and the LDS layout.
+//
+// Conventions (the reference's, include/onefft.h:26-34): data are interleaved complex float
+// (re, im), forward is exp(-2 pi i jk/n), nothing is normalised. The kernels only implement the
+// forward direction; the inverse exchanges re and im on the way in and out, which is the
+// reference's own inverse (src/onefft.c:86-88, docs/design.md:137-144).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+// Register-level helpers are also host functions, so the butterflies can be checked on a CPU.
+#define ONEFFT_HD __host__ __device__ __forceinline__
+
+namespace onefft {
+
+// One complex value, 8-byte aligned so a load or store is one dwordx2.
+struct __attribute__((aligned(8))) cf {
+    float r, i;
+};
+
+// One complex value in FP64, for twiddle products that are rounded to FP32 only once.
+struct __attribute__((aligned(16))) cd {
+    double r, i;
+};
+
+ONEFFT_HD cf mk(float r, float i) {
+    cf c;
+    c.r = r;
+    c.i = i;
+    return c;
+}
+ONEFFT_HD cf add(cf a, cf b) { return mk(a.r + b.r, a.i + b.i); }
+ONEFFT_HD cf sub(cf a, cf b) { return mk(a.r - b.r, a.i - b.i); }
+// a * w
+ONEFFT_HD cf mul(cf a, cf w) { return mk(a.r * w.r - a.i * w.i, a.r * w.i + a.i * w.r); }
+// a * (-i)
+ONEFFT_HD cf mul_mi(cf a) { return mk(a.i, -a.r); }
+// exchange re and im (the inverse transform's input and output mapping)
+ONEFFT_HD cf swap_ri(cf a) { return mk(a.i, a.r); }
+template <bool SWAP>
+ONEFFT_HD cf maybe_swap(cf a) {
+    if constexpr (SWAP) return swap_ri(a);
+    else return a;
+}
+
+ONEFFT_HD cd mul64(cd a, cd b) {
+    cd c;
+    c.r = a.r * b.r - a.i * b.i;
+    c.i = a.r * b.i + a.i * b.r;
+    return c;
+}
+ONEFFT_HD cf round32(cd a) { return mk((float)a.r, (float)a.i); }
+
+// ---- compile-time loops -------------------------------------------------------------------
+
+template <int I>
+using ic = std::integral_constant<int, I>;
+
+template <int B, int E, typename F>
+ONEFFT_HD void static_for(F &&f) {
+    if constexpr (B < E) {
+        f(ic<B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+// ---- constant twiddles w_R^E = exp(-2 pi i E / R), R a power of two up to 32 ---------------
+
+// cos(2 pi e / 32) for e = 0..8
+constexpr double kCos32[9] = {1.0,
+                              0.98078528040323044912618223613424,
+                              0.92387953251128675612818318939679,
+                              0.83146961230254523707878837761791,
+                              0.70710678118654752440084436210485,
+                              0.55557023301960222474283081394853,
+                              0.38268343236508977172845998403040,
+                              0.19509032201612826784828486847702,
+                              0.0};
+
+constexpr double cos32(int e) {
+    e &= 31;
+    return e <= 8 ? kCos32[e] : e <= 16 ? -kCos32[16 - e] : e <= 24 ? -kCos32[e - 16] : kCos32[32 - e];
+}
+constexpr double sin32(int e) { return cos32(e - 8); }
+
+// a * w_R^E with every trivial or eighth-turn case written out.
+template <int R, int E>
+ONEFFT_HD cf twc(cf a) {
+    static_assert(R >= 1 && R <= 32 && (R & (R - 1)) == 0, "radix must be a power of two up to 32");
+    constexpr int e = ((E % R) + R) % R * (32 / R);  // in 32nds of a turn
+    constexpr float h = 0.70710678118654752440f;
+    if constexpr (e == 0) {
+        return a;
+    } else if constexpr (e == 8) {
+        return mk(a.i, -a.r);  // -i
+    } else if constexpr (e == 16) {
+        return mk(-a.r, -a.i);
+    } else if constexpr (e == 24) {
+        return mk(-a.i, a.r);  // +i
+    } else if constexpr (e == 4) {
+        return mk((a.r + a.i) * h, (a.i - a.r) * h);  // (1 - i)/sqrt2
+    } else if constexpr (e == 12) {
+        return mk((a.i - a.r) * h, -(a.r + a.i) * h);  // (-1 - i)/sqrt2
+    } else if constexpr (e == 20) {
+        return mk(-(a.r + a.i) * h, (a.r - a.i) * h);  // (-1 + i)/sqrt2
+    } else if constexpr (e == 28) {
+        return mk((a.r - a.i) * h, (a.r + a.i) * h);  // (1 + i)/sqrt2
+    } else {
+        constexpr float c = (float)cos32(e), s = (float)sin32(e);
+        // (a.r + i a.i)(c - i s)
+        return mk(a.r * c + a.i * s, a.i * c - a.r * s);
+    }
+}
+
+// ---- register DFTs (forward, natural order in and out) -------------------------------------
+
+template <int R>
+ONEFFT_HD void dft(cf *v);
+
+template <>
+ONEFFT_HD void dft<1>(cf *) {}
+
+template <>
+ONEFFT_HD void dft<2>(cf *v) {
+    cf a = v[0], b = v[1];
+    v[0] = add(a, b);
+    v[1] = sub(a, b);
+}
+
+template <>
+ONEFFT_HD void dft<4>(cf *v) {
+    cf s0 = add(v[0], v[2]), d0 = sub(v[0], v[2]);
+    cf s1 = add(v[1], v[3]), d1 = mul_mi(sub(v[1], v[3]));
+    v[0] = add(s0, s1);
+    v[2] = sub(s0, s1);
+    v[1] = add(d0, d1);
+    v[3] = sub(d0, d1);
+}
+
+// R = A * B by one decimation-in-time split: x[a + A b] -> A DFTs of size B over b, twiddles
+// w_R^(a k), then B DFTs of size A over a; output index k + B * m.
+template <int R>
+ONEFFT_HD void dft(cf *v) {
+    constexpr int A = (R == 8) ? 2 : 4;
+    constexpr int B = R / A;
+    cf y[A][B];
+    static_for<0, A>([&](auto a) {
+        static_for<0, B>([&](auto b) { y[a][b] = v[a + A * b]; });
+        dft<B>(y[a]);
+        static_for<1, B>([&](auto k) { y[a][k] = twc<R, decltype(a)::value * decltype(k)::value>(y[a][k]); });
+    });
+    static_for<0, B>([&](auto k) {
+        cf z[A];
+        static_for<0, A>([&](auto a) { z[a] = y[a][k]; });
+        dft<A>(z);
+        static_for<0, A>([&](auto m) { v[k + B * m] = z[m]; });
+    });
+}
+
+// ---- Stockham stage schedule ------------------------------------------------------------------
+//
+// An N-point transform in which every thread holds P points runs in stages. Stage s has radix
+// R_s (a divisor of P) and Ns = R_0 ... R_{s-1}; each thread runs P / R_s butterflies. Butterfly
+// b reads x[b + r N / R_s], multiplies input r by w_{Ns R_s}^{r (b mod Ns)}, transforms, and
+// writes output m to (b / Ns) Ns R_s + (b mod Ns) + m Ns. After the last stage the result is in
+// natural order (Stockham autosort; Govindaraju et al., SC 2008). The smaller radix, if any,
+// comes first.
+
+__host__ __device__ constexpr int stage_count(int log2n, int log2p) {
+    return log2n <= log2p ? 1 : log2n / log2p + (log2n % log2p ? 1 : 0);
+}
+__host__ __device__ constexpr int stage_log2r(int log2n, int log2p, int s) {
+    return log2n <= log2p ? log2n : (log2n % log2p && s == 0) ? log2n % log2p : log2p;
+}
+// log2 of Ns for stage s
+__host__ __device__ constexpr int stage_log2ns(int log2n, int log2p, int s) {
+    int l = 0;
+    for (int i = 0; i < s; i++) l += stage_log2r(log2n, log2p, i);
+    return l;
+}
+// Offset (in complex entries) of stage s's twiddles in the per-size table, and the table size.
+// Stage s stores w_{Ns R}^{r k} at [(r - 1) Ns + k] for r = 1..R-1, k = 0..Ns-1.
+__host__ __device__ constexpr uint32_t stage_tw_offset(int log2n, int log2p, int s) {
+    uint32_t off = 0;
+    for (int i = 1; i < s; i++) {
+        uint32_t r = 1u << stage_log2r(log2n, log2p, i);
+        off += (r - 1) << stage_log2ns(log2n, log2p, i);
+    }
+    return off;
+}
+__host__ __device__ constexpr uint32_t stage_tw_size(int log2n, int log2p) {
+    return stage_tw_offset(log2n, log2p, stage_count(log2n, log2p));
+}
+
+// ---- LDS layout ---------------------------------------------------------------------------
+//
+// One complex of padding after every 16: stage accesses that stride by powers of two then
+// spread over the banks.
+__host__ __device__ constexpr uint32_t lds_pad(uint32_t i) { return i + (i >> 4); }
+__host__ __device__ constexpr uint32_t lds_span(uint32_t n) { return n + (n >> 4); }
+
+}  // namespace onefft

@@ -1,0 +1,74 @@
+// Host-side view of the kernels: launch parameters, the per-size kernel configuration (shared
+// with the device code so the twiddle tables the host builds match what the kernels read), and
+// the launch functions. No HIP device code here.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace onefft {
+
+struct cf;
+struct cd;
+
+// Largest size that runs as one LDS-resident pass per transform.
+constexpr int kMaxSinglePassLog2 = 14;
+
+// Points held per thread by the single-pass kernel of size 2^log2n.
+constexpr int batch_log2p(int log2n) { return log2n <= 4 ? log2n : 4; }
+// Threads per workgroup of the single-pass kernel of size 2^log2n.
+constexpr int batch_wg(int log2n) {
+    return log2n <= 4 ? 256 : ((1 << (log2n - 4)) > 256 ? (1 << (log2n - 4)) : 256);
+}
+// Transforms per workgroup.
+constexpr int batch_per_wg(int log2n) { return batch_wg(log2n) / (1 << (log2n - batch_log2p(log2n))); }
+
+struct BatchArgs {
+    const cf *in;
+    cf *out;
+    const cf *tw;        // stage twiddles of this size (stage_tw_size(log2n, batch_log2p(log2n)) entries)
+    uint64_t in_dist;    // complex values between transforms
+    uint64_t out_dist;
+    uint64_t batch;
+};
+
+// One launch: `batch` transforms of 2^log2n points, 0 <= log2n <= kMaxSinglePassLog2.
+hipError_t launch_batch(int log2n, bool inverse, const BatchArgs &a, hipStream_t stream);
+
+// ---- multi-pass (four-step) transforms, n > 2^kMaxSinglePassLog2 ----------------------------
+
+constexpr int kMaxPasses = 4;
+constexpr int kPassMinLog2 = 4;   // smallest sub-transform of a pass
+constexpr int kPassMaxLog2 = 10;  // largest sub-transform of a pass
+
+// Points per thread and columns per workgroup of the pass kernel for a 2^log2nf sub-transform.
+constexpr int pass_log2p(int log2nf) { return log2nf < 4 ? log2nf : 4; }
+constexpr int pass_cols(int log2nf) { return log2nf >= 10 ? 8 : 16; }
+constexpr int pass_wg(int log2nf) { return pass_cols(log2nf) << (log2nf - pass_log2p(log2nf)); }
+
+struct PassArgs {
+    const cf *in;
+    cf *out;
+    const cf *tw;         // stage twiddles of the sub-transform size
+    const cd *tw_lo;      // w_n^i, i < 2^lo_bits (FP64)
+    const cd *tw_hi;      // w_n^(i 2^lo_bits), i < n / 2^lo_bits (FP64)
+    uint64_t in_dist;     // complex values between transforms of a batch
+    uint64_t out_dist;
+    uint32_t lo_bits;
+    uint32_t log2n;
+    uint32_t log2m;       // low-index extent M_f of this pass (0 for the first pass)
+    uint32_t tw_shift;    // log2(n / (N_f M_f)): scales the pass twiddle exponent to w_n
+    // first pass only: output row of input column J; digits j_1.. of J (j_1 fastest)
+    uint32_t ndig;
+    uint32_t dig_log2[kMaxPasses];   // log2 N_i of each digit
+    uint32_t dig_shift[kMaxPasses];  // log2 of each digit's weight in the row index
+    uint32_t columns;     // n / N_f
+};
+
+// One pass of a multi-pass transform: first = reads the input (out of place), else in place on
+// `out`. swap_in / swap_out exchange re and im on load / store (the inverse).
+hipError_t launch_pass(int log2nf, bool first, bool swap_in, bool swap_out, const PassArgs &a, uint64_t batch,
+                       hipStream_t stream);
+
+}  // namespace onefft

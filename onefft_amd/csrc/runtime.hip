@@ -1,0 +1,428 @@
+// Host runtime behind include/onefft.h: argument checks, plans with device-resident twiddle
+// tables, the write-once plan cache, the host-pointer staging path and the device-pointer,
+// batched entry points.
+//
+// Reference counterparts (src/onefft.c): plan creation and validation :1111-1136, the cache
+// :1144-1189, the public transforms :1138-1142 and :1169-1181, the twiddle function :112-141.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/onefft.h"
+#include "fft_common.hpp"
+#include "kernels.hpp"
+#include "runtime.hpp"
+
+using onefft::cd;
+using onefft::cf;
+
+namespace {
+
+constexpr unsigned kMaxLog2 = 30;  // the reference's limit (src/onefft.c:1117-1121)
+
+// ---- twiddles -------------------------------------------------------------------------------
+//
+// exp(-2 pi i k / 2^log2size) in FP64. The phase k / 2^log2size of a turn is held exactly as a
+// 0.64 fixed-point fraction; the angle is folded into the first octant before sin/cos and the
+// quadrant is applied exactly, so values are exactly symmetric and exact at multiples of an
+// eighth turn. Same steps as the reference's onefft_root_double (src/onefft.c:112-135), which
+// fixes the twiddle values bit for bit once rounded to FP32.
+void root64(uint64_t k, unsigned log2size, double *re, double *im) {
+    const uint64_t phase = log2size ? k << (64 - log2size) : 0;
+    const unsigned quadrant = (unsigned)(phase >> 62);
+    const uint64_t in_quadrant = phase & ((uint64_t(1) << 62) - 1);
+    const double unit = 6.283185307179586476925286766559 / 18446744073709551616.0;  // radians per 2^-64 turn
+    double c, s;
+    if (in_quadrant > (uint64_t(1) << 61)) {
+        const double a = (double)((uint64_t(1) << 62) - in_quadrant) * unit;
+        c = std::sin(a);
+        s = std::cos(a);
+    } else {
+        const double a = (double)in_quadrant * unit;
+        c = std::cos(a);
+        s = std::sin(a);
+    }
+    switch (quadrant) {  // (c - i s) (-i)^quadrant
+    case 0: *re = c, *im = -s; break;
+    case 1: *re = -s, *im = -c; break;
+    case 2: *re = -c, *im = s; break;
+    default: *re = s, *im = c; break;
+    }
+}
+
+cf root32(uint64_t k, unsigned log2size) {
+    double re, im;
+    root64(k, log2size, &re, &im);
+    cf c;
+    c.r = (float)re;
+    c.i = (float)im;
+    return c;
+}
+
+// Stage twiddles of an in-LDS transform of 2^log2n points, P = 2^log2p points per thread, in
+// the order the kernels read them (fft_common.hpp: stage_tw_offset).
+void fill_stage_table(int log2n, int log2p, cf *table) {
+    const int stages = onefft::stage_count(log2n, log2p);
+    for (int s = 1; s < stages; s++) {
+        const uint32_t off = onefft::stage_tw_offset(log2n, log2p, s);
+        const int lr = onefft::stage_log2r(log2n, log2p, s);
+        const int lns = onefft::stage_log2ns(log2n, log2p, s);
+        const uint32_t r_count = 1u << lr, ns = 1u << lns;
+        for (uint32_t r = 1; r < r_count; r++)
+            for (uint32_t k = 0; k < ns; k++) table[off + (r - 1) * ns + k] = root32((uint64_t)r * k, lns + lr);
+    }
+}
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Pass split of a multi-pass size: log2 N_1 .. log2 N_p (N_1 is transformed last). Even split
+// into passes of at most 2^9 points, the larger factors first in execution order. The
+// environment variable ONEFFT_PASS_SPLIT ("a,b[,c[,d]]", log2 N_1 first) overrides it for sizes
+// whose factors it matches, to sweep splits without rebuilding (the reference's analogue is its
+// ONEFFT_X86_TUNING build switch, src/onefft.c:652-697).
+int choose_split(unsigned log2n, int *factors) {
+    if (const char *env = std::getenv("ONEFFT_PASS_SPLIT")) {
+        int f[onefft::kMaxPasses], p = 0, sum = 0;
+        const char *s = env;
+        while (*s && p < onefft::kMaxPasses) {
+            char *end;
+            long v = std::strtol(s, &end, 10);
+            if (end == s) break;
+            f[p++] = (int)v;
+            sum += (int)v;
+            s = *end == ',' ? end + 1 : end;
+        }
+        bool ok = p >= 2 && sum == (int)log2n;
+        for (int i = 0; i < p && ok; i++) ok = f[i] >= onefft::kPassMinLog2 && f[i] <= onefft::kPassMaxLog2;
+        if (ok) {
+            for (int i = 0; i < p; i++) factors[i] = f[i];
+            return p;
+        }
+    }
+    int p = (int)((log2n + 8) / 9);
+    if (p < 2) p = 2;
+    for (int i = 0; i < p; i++) factors[i] = (int)log2n / p + (i >= p - (int)log2n % p ? 1 : 0);
+    return p;
+}
+
+}  // namespace
+
+// ---- plans ----------------------------------------------------------------------------------
+
+struct PassPlan {
+    int log2nf;
+    onefft::PassArgs args;  // pointers to data filled in per call
+};
+
+struct onefft_plan {
+    size_t n = 0;
+    unsigned log2n = 0;
+    int device = 0;
+    void *dmem = nullptr;  // the one device allocation holding every table
+    const cf *tw_single = nullptr;
+    int passes = 0;  // 0: single LDS pass
+    PassPlan pass[onefft::kMaxPasses];  // in execution order
+};
+
+namespace onefft {
+
+static thread_local std::string t_error;
+
+void set_error(const char *what, hipError_t e) { t_error = std::string(what) + ": " + hipGetErrorString(e); }
+void set_error(const char *what) { t_error = what; }
+
+bool size_log2(size_t n, unsigned *log2n) {
+    if (n == 0 || (n & (n - 1)) != 0) return false;
+    unsigned l = 0;
+    while (((size_t)1 << l) < n) l++;
+    if (l > kMaxLog2) return false;
+    *log2n = l;
+    return true;
+}
+
+namespace {
+
+// RAII: make the plan's device current for the duration of a call.
+struct DeviceGuard {
+    int previous = -1;
+    bool ok = true;
+    explicit DeviceGuard(int device) {
+        int cur = 0;
+        if (hipGetDevice(&cur) != hipSuccess) {
+            ok = false;
+            return;
+        }
+        if (cur != device) {
+            if (hipSetDevice(device) != hipSuccess) ok = false;
+            else previous = cur;
+        }
+    }
+    ~DeviceGuard() {
+        if (previous >= 0) (void)hipSetDevice(previous);
+    }
+};
+
+}  // namespace
+
+// Launches the transform of `batch` arrays on `stream`; data pointers are device memory.
+int execute_device(const onefft_plan *plan, const cf *d_in, cf *d_out, uint64_t batch, uint64_t in_dist,
+                   uint64_t out_dist, bool inverse, hipStream_t stream) {
+    if (batch == 0) return 0;
+    if (plan->passes == 0) {
+        BatchArgs a{d_in, d_out, plan->tw_single, in_dist, out_dist, batch};
+        hipError_t e = launch_batch((int)plan->log2n, inverse, a, stream);
+        if (e != hipSuccess) {
+            set_error("single-pass launch", e);
+            return -1;
+        }
+        return 0;
+    }
+    for (uint64_t done = 0; done < batch;) {
+        const uint64_t chunk = batch - done < 65535 ? batch - done : 65535;
+        for (int i = 0; i < plan->passes; i++) {
+            PassArgs a = plan->pass[i].args;
+            const bool first = i == 0, last = i == plan->passes - 1;
+            a.in = first ? d_in + done * in_dist : d_out + done * out_dist;
+            a.out = d_out + done * out_dist;
+            a.in_dist = first ? in_dist : out_dist;
+            a.out_dist = out_dist;
+            hipError_t e = launch_pass(plan->pass[i].log2nf, first, first && inverse, last && inverse, a, chunk, stream);
+            if (e != hipSuccess) {
+                set_error("pass launch", e);
+                return -1;
+            }
+        }
+        done += chunk;
+    }
+    return 0;
+}
+
+}  // namespace onefft
+
+using namespace onefft;
+
+extern "C" onefft_plan *onefft_plan_create(size_t n) {
+    unsigned log2n;
+    if (!size_log2(n, &log2n)) return nullptr;
+    onefft_plan *plan = new (std::nothrow) onefft_plan;
+    if (!plan) return nullptr;
+    plan->n = n;
+    plan->log2n = log2n;
+    if (hipGetDevice(&plan->device) != hipSuccess) {
+        set_error("no HIP device");
+        delete plan;
+        return nullptr;
+    }
+
+    // host image of the device tables, then one allocation and one copy
+    std::vector<unsigned char> image;
+    auto reserve = [&](size_t bytes) {
+        size_t off = align_up(image.size(), 256);
+        image.resize(off + bytes);
+        return off;
+    };
+    size_t off_single = 0, off_stage[kMaxPasses] = {}, off_lo = 0, off_hi = 0;
+    if (log2n <= (unsigned)kMaxSinglePassLog2) {
+        const int lp = batch_log2p((int)log2n);
+        const uint32_t entries = stage_tw_size((int)log2n, lp);
+        off_single = reserve(entries * sizeof(cf) + sizeof(cf));
+        fill_stage_table((int)log2n, lp, reinterpret_cast<cf *>(image.data() + off_single));
+    } else {
+        int factors[kMaxPasses];
+        const int p = choose_split(log2n, factors);
+        plan->passes = p;
+        // two-level FP64 table of w_n
+        const unsigned lo_bits = (log2n + 1) / 2;
+        const size_t lo_count = (size_t)1 << lo_bits, hi_count = (size_t)1 << (log2n - lo_bits);
+        off_lo = reserve(lo_count * sizeof(cd));
+        off_hi = reserve(hi_count * sizeof(cd));
+        {
+            cd *lo = reinterpret_cast<cd *>(image.data() + off_lo);
+            for (size_t i = 0; i < lo_count; i++) root64(i, log2n, &lo[i].r, &lo[i].i);
+            cd *hi = reinterpret_cast<cd *>(image.data() + off_hi);
+            for (size_t i = 0; i < hi_count; i++) root64((uint64_t)i << lo_bits, log2n, &hi[i].r, &hi[i].i);
+        }
+        // execution order: factor f = p, p-1, ..., 1 (factors[f-1])
+        for (int i = 0; i < p; i++) {
+            const int f = p - i;  // 1-based factor index
+            const int lf = factors[f - 1];
+            const int lp = pass_log2p(lf);
+            off_stage[i] = reserve(stage_tw_size(lf, lp) * sizeof(cf) + sizeof(cf));
+            fill_stage_table(lf, lp, reinterpret_cast<cf *>(image.data() + off_stage[i]));
+            PassArgs &a = plan->pass[i].args;
+            std::memset(&a, 0, sizeof(a));
+            plan->pass[i].log2nf = lf;
+            unsigned log2m = 0;
+            for (int l = f + 1; l <= p; l++) log2m += (unsigned)factors[l - 1];
+            a.lo_bits = lo_bits;
+            a.log2n = log2n;
+            a.log2m = log2m;
+            a.tw_shift = log2n - log2m - (unsigned)lf;
+            a.columns = (uint32_t)(n >> lf);
+            if (i == 0) {
+                a.ndig = (uint32_t)(p - 1);
+                for (int d = 0; d < p - 1; d++) {  // digit j_{d+1}
+                    a.dig_log2[d] = (uint32_t)factors[d];
+                    uint32_t shift = 0;
+                    for (int l = d + 2; l <= p - 1; l++) shift += (uint32_t)factors[l - 1];
+                    a.dig_shift[d] = shift;
+                }
+            }
+        }
+    }
+
+    if (hipMalloc(&plan->dmem, image.size()) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("plan table allocation failed");
+        delete plan;
+        return nullptr;
+    }
+    if (hipMemcpy(plan->dmem, image.data(), image.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("plan table upload failed");
+        (void)hipFree(plan->dmem);
+        delete plan;
+        return nullptr;
+    }
+    auto dev = [&](size_t off) { return reinterpret_cast<unsigned char *>(plan->dmem) + off; };
+    if (plan->passes == 0) {
+        plan->tw_single = reinterpret_cast<const cf *>(dev(off_single));
+    } else {
+        for (int i = 0; i < plan->passes; i++) {
+            PassArgs &a = plan->pass[i].args;
+            a.tw = reinterpret_cast<const cf *>(dev(off_stage[i]));
+            a.tw_lo = reinterpret_cast<const cd *>(dev(off_lo));
+            a.tw_hi = reinterpret_cast<const cd *>(dev(off_hi));
+        }
+    }
+    staging_reserve(plan->device, n);
+    return plan;
+}
+
+extern "C" void onefft_plan_destroy(onefft_plan *plan) {
+    if (!plan) return;
+    if (plan->dmem) {
+        DeviceGuard g(plan->device);
+        (void)hipFree(plan->dmem);
+    }
+    delete plan;
+}
+
+extern "C" size_t onefft_plan_size(const onefft_plan *plan) { return plan ? plan->n : 0; }
+
+extern "C" const char *onefft_implementation(void) { return "hip-gfx950"; }
+
+extern "C" const char *onefft_last_error(void) { return t_error.c_str(); }
+
+static int run_sync(const onefft_plan *plan, const float *in, float *out, bool inverse) {
+    if (!plan || !in || !out) {
+        set_error("null plan or buffer");
+        return -1;
+    }
+    DeviceGuard g(plan->device);
+    if (!g.ok) {
+        set_error("cannot select the plan's device");
+        return -1;
+    }
+    return execute_host_call(plan, in, out, inverse);
+}
+
+extern "C" void onefft_plan_forward(const onefft_plan *plan, const float *in, float *out) {
+    (void)run_sync(plan, in, out, false);
+}
+extern "C" void onefft_plan_inverse(const onefft_plan *plan, const float *in, float *out) {
+    (void)run_sync(plan, in, out, true);
+}
+
+static int run_device(const onefft_plan *plan, const float *d_in, float *d_out, size_t batch, size_t in_dist,
+                      size_t out_dist, bool inverse, hipStream_t stream) {
+    if (!plan || !d_in || !d_out) {
+        set_error("null plan or buffer");
+        return -1;
+    }
+    if (((uintptr_t)d_in | (uintptr_t)d_out) & 7) {
+        set_error("device buffers must be 8-byte aligned");
+        return -1;
+    }
+    if (batch > 1 && (in_dist < plan->n || out_dist < plan->n)) {
+        set_error("batch distance smaller than the transform size");
+        return -1;
+    }
+    DeviceGuard g(plan->device);
+    if (!g.ok) {
+        set_error("cannot select the plan's device");
+        return -1;
+    }
+    return execute_device(plan, reinterpret_cast<const cf *>(d_in), reinterpret_cast<cf *>(d_out), batch, in_dist,
+                          out_dist, inverse, stream);
+}
+
+extern "C" int onefft_plan_forward_device(const onefft_plan *plan, const float *d_in, float *d_out,
+                                          struct ihipStream_t *stream) {
+    return run_device(plan, d_in, d_out, 1, plan ? plan->n : 0, plan ? plan->n : 0, false, stream);
+}
+extern "C" int onefft_plan_inverse_device(const onefft_plan *plan, const float *d_in, float *d_out,
+                                          struct ihipStream_t *stream) {
+    return run_device(plan, d_in, d_out, 1, plan ? plan->n : 0, plan ? plan->n : 0, true, stream);
+}
+extern "C" int onefft_plan_forward_batch(const onefft_plan *plan, const float *d_in, float *d_out, size_t batch,
+                                         size_t in_dist, size_t out_dist, struct ihipStream_t *stream) {
+    return run_device(plan, d_in, d_out, batch, in_dist, out_dist, false, stream);
+}
+extern "C" int onefft_plan_inverse_batch(const onefft_plan *plan, const float *d_in, float *d_out, size_t batch,
+                                         size_t in_dist, size_t out_dist, struct ihipStream_t *stream) {
+    return run_device(plan, d_in, d_out, batch, in_dist, out_dist, true, stream);
+}
+
+// ---- the plan cache (src/onefft.c:1144-1189) -------------------------------------------------
+//
+// One slot per log2 n, written once with a compare-and-swap and only read afterwards: a warm call
+// is one acquire load. Two threads that create the same plan at once both build it; the loser
+// destroys its copy.
+
+static std::atomic<onefft_plan *> g_cache[64];
+
+static const onefft_plan *cache_get(size_t n) {
+    unsigned log2n;
+    if (!size_log2(n, &log2n)) return nullptr;
+    std::atomic<onefft_plan *> &slot = g_cache[log2n];
+    onefft_plan *plan = slot.load(std::memory_order_acquire);
+    if (plan) return plan;
+    plan = onefft_plan_create(n);
+    if (!plan) return nullptr;
+    onefft_plan *expected = nullptr;
+    if (!slot.compare_exchange_strong(expected, plan, std::memory_order_acq_rel, std::memory_order_acquire)) {
+        onefft_plan_destroy(plan);
+        plan = expected;
+    }
+    return plan;
+}
+
+extern "C" int onefft_forward(const float *in, float *out, size_t n) {
+    const onefft_plan *plan = cache_get(n);
+    if (!plan) return -1;
+    return run_sync(plan, in, out, false);
+}
+
+extern "C" int onefft_inverse(const float *in, float *out, size_t n) {
+    const onefft_plan *plan = cache_get(n);
+    if (!plan) return -1;
+    return run_sync(plan, in, out, true);
+}
+
+extern "C" int onefft_cache_prepare(size_t n) {
+    const onefft_plan *plan = cache_get(n);
+    if (!plan) return -1;
+    return staging_reserve(plan->device, n) ? 0 : -1;
+}
+
+extern "C" void onefft_cache_clear(void) {
+    for (auto &slot : g_cache) onefft_plan_destroy(slot.exchange(nullptr, std::memory_order_acq_rel));
+}

@@ -1,0 +1,35 @@
+// Internal interfaces between the runtime's translation units (not part of the public API).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "kernels.hpp"
+
+struct onefft_plan;
+
+namespace onefft {
+
+// n -> log2 n for a power of two from 1 to 2^30, else false.
+bool size_log2(size_t n, unsigned *log2n);
+
+// Record the message returned by onefft_last_error() on this thread.
+void set_error(const char *what, hipError_t e);
+void set_error(const char *what);
+
+// Launch `batch` transforms on device memory, asynchronously on `stream`.
+int execute_device(const onefft_plan *plan, const cf *d_in, cf *d_out, uint64_t batch, uint64_t in_dist,
+                   uint64_t out_dist, bool inverse, hipStream_t stream);
+
+// Make sure the calling thread has a stream and device staging for transforms of n points on
+// `device`. false on failure.
+bool staging_reserve(int device, size_t n);
+
+// The synchronous drop-in call: host or device pointers of any alignment.
+int execute_host_call(const onefft_plan *plan, const float *in, float *out, bool inverse);
+
+// The calling thread's stream on `device` (created on first use), or nullptr on failure.
+hipStream_t thread_stream(int device);
+
+}  // namespace onefft

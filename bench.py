@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="approximate CPU-baseline budget")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="only the timed transform launches plus one 1 GiB device copy (for counter runs)")
     return ap.parse_args()
 
 
@@ -117,6 +119,12 @@ def main():
 
     flops = 5.0 * N * math.log2(N) * BATCH
     alg_bytes = 16.0 * N * BATCH
+    if args.kernel_only:
+        z = torch.empty_like(x)
+        z.copy_(x)  # a plain copy of known size: 512 MiB read, 512 MiB written
+        torch.cuda.synchronize()
+        print(json.dumps({"kernel_only": True, "ms_per_step": ms, "gb_s": alg_bytes / (ms * 1e-3) / 1e9}), flush=True)
+        return
     gflops = flops / (ms * 1e-3) / 1e9
     achieved = alg_bytes / (ms * 1e-3) / 1e9
 

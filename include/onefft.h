@@ -101,6 +101,44 @@ int onefft_plan_forward_batch(const onefft_plan *plan, const float *d_in, float 
 int onefft_plan_inverse_batch(const onefft_plan *plan, const float *d_in, float *d_out, size_t batch,
                               size_t in_dist, size_t out_dist, struct ihipStream_t *stream);
 
+/* ---- ragged batches: many transforms of mixed sizes in one launch ---------------------------
+ * (no reference counterpart: with the reference each array is its own onefft_forward/inverse
+ * call, include/onefft.h:76-77). */
+
+/* One array of a ragged batch: 2^log2n complex values (log2n <= 20) at `offset` complex values
+ * from the start of both the input and the output buffer, transformed forward (inverse = 0) or
+ * inverse (1). */
+typedef struct {
+    uint64_t offset;
+    uint32_t log2n;
+    uint32_t inverse;
+} onefft_ragged_item;
+
+typedef struct onefft_ragged_plan onefft_ragged_plan;
+
+/* Validates the items (sizes, directions, no two overlapping), builds the task queue and the
+ * tables on the current device. NULL on invalid items or allocation failure. */
+onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_item *items, size_t count);
+void onefft_ragged_plan_destroy(onefft_ragged_plan *plan);
+
+/* Complex values the input and output buffers must hold (largest offset + size), and the number
+ * of tasks one execution runs. */
+size_t onefft_ragged_extent(const onefft_ragged_plan *plan);
+size_t onefft_ragged_task_count(const onefft_ragged_plan *plan);
+
+/* Transforms every item in one persistent-kernel launch, asynchronously on `stream`. d_in is not
+ * modified. A ragged plan runs one execution at a time: executions of one plan must be ordered
+ * (same stream or synchronised). Returns 0 or -1. */
+int onefft_ragged_execute(const onefft_ragged_plan *plan, const float *d_in, float *d_out, struct ihipStream_t *stream);
+
+/* The same result with one device plan call per item, for comparison. */
+int onefft_ragged_execute_per_item(const onefft_ragged_plan *plan, const float *d_in, float *d_out,
+                                   struct ihipStream_t *stream);
+
+/* Waits for `stream` and reports whether the last onefft_ragged_execute of this plan completed
+ * correctly: 0, or -1 with onefft_last_error() set. */
+int onefft_ragged_wait(const onefft_ragged_plan *plan, struct ihipStream_t *stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 

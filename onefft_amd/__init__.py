@@ -40,7 +40,20 @@ SIGNATURES = {
     "onefft_plan_inverse_device": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "onefft_plan_forward_batch": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t, c_size_t, c_size_t, c_void_p]),
     "onefft_plan_inverse_batch": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t, c_size_t, c_size_t, c_void_p]),
+    "onefft_ragged_plan_create": (c_void_p, [c_void_p, c_size_t]),
+    "onefft_ragged_plan_destroy": (None, [c_void_p]),
+    "onefft_ragged_extent": (c_size_t, [c_void_p]),
+    "onefft_ragged_task_count": (c_size_t, [c_void_p]),
+    "onefft_ragged_execute": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "onefft_ragged_execute_per_item": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "onefft_ragged_wait": (c_int, [c_void_p, c_void_p]),
 }
+
+
+class RaggedItem(ctypes.Structure):
+    """onefft_ragged_item: (offset in complex values, log2 n, inverse)."""
+
+    _fields_ = [("offset", ctypes.c_uint64), ("log2n", ctypes.c_uint32), ("inverse", ctypes.c_uint32)]
 
 _lib = None
 
@@ -119,6 +132,74 @@ class Plan:
                   stream: int = 0) -> None:
         fn = lib().onefft_plan_inverse_batch if inverse else lib().onefft_plan_forward_batch
         _check(fn(self._p, src, dst, batch, in_dist, out_dist, stream), "onefft_plan_*_batch")
+
+
+class RaggedPlan:
+    """An onefft_ragged_plan over items given as (offset, log2n, inverse) triples."""
+
+    def __init__(self, items):
+        import numpy as np
+
+        arr = np.asarray(items, dtype=np.uint64).reshape(-1, 3)
+        self.count = arr.shape[0]
+        buf = (RaggedItem * max(self.count, 1))()
+        for i, (off, l, inv) in enumerate(arr.tolist()):
+            buf[i] = RaggedItem(off, l, inv)
+        self._items = buf
+        self._p = lib().onefft_ragged_plan_create(ctypes.addressof(buf), self.count)
+        if not self._p:
+            raise OneFFTError(f"onefft_ragged_plan_create failed: {last_error()}")
+
+    @property
+    def extent(self) -> int:
+        return lib().onefft_ragged_extent(self._p)
+
+    @property
+    def tasks(self) -> int:
+        return lib().onefft_ragged_task_count(self._p)
+
+    def execute(self, src: int, dst: int, stream: int = 0) -> None:
+        _check(lib().onefft_ragged_execute(self._p, src, dst, stream), "onefft_ragged_execute")
+
+    def execute_per_item(self, src: int, dst: int, stream: int = 0) -> None:
+        _check(lib().onefft_ragged_execute_per_item(self._p, src, dst, stream), "onefft_ragged_execute_per_item")
+
+    def wait(self, stream: int = 0) -> None:
+        _check(lib().onefft_ragged_wait(self._p, stream), "onefft_ragged_wait")
+
+    def close(self) -> None:
+        if self._p:
+            lib().onefft_ragged_plan_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ragged_items(seed: int, count: int, lo: int = 4, hi: int = 20, max_bytes: int | None = None):
+    """The ragged workload's items: log2 n uniform on lo..hi and the direction a fair coin, both
+    from SplitMix64(seed) (two draws per item), packed back to back. With max_bytes, the longest
+    prefix whose input fits in max_bytes. Returns a list of (offset, log2n, inverse)."""
+    items, off, s = [], 0, seed & (2**64 - 1)
+    M = 2**64 - 1
+    for _ in range(count):
+        vals = []
+        for _ in range(2):
+            s = (s + 0x9E3779B97F4A7C15) & M
+            z = s
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+            vals.append(z ^ (z >> 31))
+        log2n = lo + int((vals[0] >> 11) * (hi - lo + 1) >> 53)
+        inverse = int(vals[1] >> 63)
+        if max_bytes is not None and (off + (1 << log2n)) * 8 > max_bytes:
+            break
+        items.append((off, log2n, inverse))
+        off += 1 << log2n
+    return items
 
 
 _plans: dict[int, Plan] = {}

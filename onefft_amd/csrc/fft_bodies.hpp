@@ -1,0 +1,228 @@
+// The two transform bodies every kernel is built from, as device functions so the standalone
+// kernels (k_batch.hip, k_pass.hip) and the persistent ragged kernel (k_ragged.hip) run the same
+// code:
+//
+//   single_body  one whole transform of up to 2^14 points in registers and LDS (Stockham stages,
+//                radix 16 with a smaller first radix where needed);
+//   pass_body    one tile of C adjacent columns of one pass of a multi-pass transform: an
+//                N_f-point sub-transform per column, with the pass twiddles formed in FP64.
+//
+// Every thread of the workgroup must call a body with the same template arguments: the bodies
+// contain workgroup barriers.
+#pragma once
+
+#include "fft_common.hpp"
+#include "kernels.hpp"
+
+namespace onefft {
+
+// ---- single pass ------------------------------------------------------------------------------
+//
+// Thread j of the transform (0 <= j < 2^LOG2N / P) works on src/dst; `live` false skips the global
+// loads and stores but keeps the thread in every barrier. L is the transform's LDS region of
+// lds_span(2^LOG2N) values (unused when the transform fits one thread).
+template <int LOG2N, bool INV>
+__device__ __forceinline__ void single_body(const cf *__restrict__ src, cf *__restrict__ dst, bool live,
+                                            const cf *__restrict__ tw, cf *L, uint32_t j) {
+    constexpr int LOG2P = batch_log2p(LOG2N);
+    constexpr int N = 1 << LOG2N;
+    constexpr int P = 1 << LOG2P;
+    constexpr uint32_t T = N / P;
+    constexpr int S = stage_count(LOG2N, LOG2P);
+    cf v[P];
+
+    static_for<0, S>([&](auto sI) {
+        constexpr int s = decltype(sI)::value;
+        constexpr int LR = stage_log2r(LOG2N, LOG2P, s);
+        constexpr int R = 1 << LR;
+        constexpr int LNS = stage_log2ns(LOG2N, LOG2P, s);
+        constexpr uint32_t NS = 1u << LNS;
+        constexpr int BF = P / R;  // butterflies per thread
+        constexpr uint32_t STRIDE = N / R;
+        constexpr uint32_t TWO = stage_tw_offset(LOG2N, LOG2P, s);
+
+        if constexpr (s == 0) {
+            if (live) {
+                static_for<0, BF>([&](auto q) {
+                    static_for<0, R>([&](auto r) {
+                        constexpr int qq = decltype(q)::value, rr = decltype(r)::value;
+                        v[qq * R + rr] = maybe_swap<INV>(src[j + T * qq + rr * STRIDE]);
+                    });
+                });
+            } else {
+                static_for<0, P>([&](auto i) { v[i] = mk(0.f, 0.f); });
+            }
+        } else {
+            static_for<0, BF>([&](auto q) {
+                static_for<0, R>([&](auto r) {
+                    constexpr int qq = decltype(q)::value, rr = decltype(r)::value;
+                    v[qq * R + rr] = L[lds_pad(j + T * qq + rr * STRIDE)];
+                });
+            });
+        }
+
+        static_for<0, BF>([&](auto q) {
+            constexpr int qq = decltype(q)::value;
+            if constexpr (s > 0) {
+                const uint32_t k = (j + T * qq) & (NS - 1);
+                static_for<1, R>([&](auto r) {
+                    constexpr int rr = decltype(r)::value;
+                    v[qq * R + rr] = mul(v[qq * R + rr], tw[TWO + (rr - 1) * NS + k]);
+                });
+            }
+            dft<R>(&v[qq * R]);
+        });
+
+        if constexpr (s == S - 1) {
+            // NS * R == N here, so output m of butterfly b goes to b + m N / R
+            if (live) {
+                static_for<0, BF>([&](auto q) {
+                    static_for<0, R>([&](auto m) {
+                        constexpr int qq = decltype(q)::value, mm = decltype(m)::value;
+                        dst[j + T * qq + mm * STRIDE] = maybe_swap<INV>(v[qq * R + mm]);
+                    });
+                });
+            }
+        } else {
+            if constexpr (s > 0) __syncthreads();  // every read of this stage is done
+            static_for<0, BF>([&](auto q) {
+                constexpr int qq = decltype(q)::value;
+                const uint32_t b = j + T * qq;
+                const uint32_t base = ((b >> LNS) << (LNS + LR)) + (b & (NS - 1));
+                static_for<0, R>([&](auto m) {
+                    constexpr int mm = decltype(m)::value;
+                    L[lds_pad(base + mm * NS)] = v[qq * R + mm];
+                });
+            });
+            __syncthreads();
+        }
+    });
+}
+
+// ---- one tile of one pass of a multi-pass transform -------------------------------------------
+//
+// Columns tile*C .. tile*C + C-1 of the pass described by `a` (see k_pass.hip for the index
+// scheme), on the transform at src/dst. Thread t of the C * 2^LOG2NF / P threads handles column
+// t % C. lds holds C * pass_span(LOG2NF) values.
+__host__ __device__ constexpr uint32_t pass_span(int log2nf) { return lds_span(1u << log2nf) | 1; }
+
+template <int LOG2NF, int C, bool FIRST, bool SWAP_IN, bool SWAP_OUT>
+__device__ __forceinline__ void pass_body(const PassArgs &a, const cf *__restrict__ src, cf *__restrict__ dst,
+                                          uint32_t tile, cf *lds, uint32_t t) {
+    constexpr int LOG2P = pass_log2p(LOG2NF);
+    constexpr int NF = 1 << LOG2NF;
+    constexpr int P = 1 << LOG2P;
+    constexpr uint32_t T = NF / P;  // threads per column
+    constexpr int S = stage_count(LOG2NF, LOG2P);
+    constexpr uint32_t SPAN = pass_span(LOG2NF);  // odd: the C columns start on different banks
+
+    const uint32_t c = t % C;
+    const uint32_t j = t / C;
+    const uint32_t col = tile * C + c;
+    cf *L = lds + c * SPAN;
+
+    uint32_t in_base, in_step, out_base, out_step, low = 0;
+    if constexpr (FIRST) {
+        in_base = col;
+        in_step = a.columns;
+        uint32_t row = 0, rest = col;
+        static_for<0, kMaxPasses - 1>([&](auto d) {  // static indexes: the arguments stay in registers
+            if (decltype(d)::value < a.ndig) {
+                row += (rest & ((1u << a.dig_log2[d]) - 1)) << a.dig_shift[d];
+                rest >>= a.dig_log2[d];
+            }
+        });
+        out_base = row << LOG2NF;
+        out_step = 1;
+    } else {
+        low = col & ((1u << a.log2m) - 1);
+        in_base = ((col >> a.log2m) << (a.log2m + LOG2NF)) + low;
+        in_step = 1u << a.log2m;
+        out_base = in_base;
+        out_step = in_step;
+    }
+
+    cf v[P];
+    static_for<0, S>([&](auto sI) {
+        constexpr int s = decltype(sI)::value;
+        constexpr int LR = stage_log2r(LOG2NF, LOG2P, s);
+        constexpr int R = 1 << LR;
+        constexpr int LNS = stage_log2ns(LOG2NF, LOG2P, s);
+        constexpr uint32_t NS = 1u << LNS;
+        constexpr int BF = P / R;
+        constexpr uint32_t STRIDE = NF / R;
+        constexpr uint32_t TWO = stage_tw_offset(LOG2NF, LOG2P, s);
+
+        if constexpr (s == 0) {
+            static_for<0, BF>([&](auto q) {
+                static_for<0, R>([&](auto r) {
+                    constexpr int qq = decltype(q)::value, rr = decltype(r)::value;
+                    v[qq * R + rr] = maybe_swap<SWAP_IN>(src[in_base + (j + T * qq + rr * STRIDE) * in_step]);
+                });
+            });
+            if constexpr (!FIRST) {
+                // element e = j + T q + STRIDE r gets w_n^((e low) << tw_shift), formed in FP64
+                const uint32_t lo_mask = (1u << a.lo_bits) - 1;
+                auto W = [&](uint32_t x) {
+                    x <<= a.tw_shift;
+                    return mul64(a.tw_hi[x >> a.lo_bits], a.tw_lo[x & lo_mask]);
+                };
+                cd wq = W(j * low);
+                const cd step_q = W(T * low);
+                const cd step_r = W(STRIDE * low);
+                static_for<0, BF>([&](auto q) {
+                    constexpr int qq = decltype(q)::value;
+                    cd w = wq;
+                    static_for<0, R>([&](auto r) {
+                        constexpr int rr = decltype(r)::value;
+                        v[qq * R + rr] = mul(v[qq * R + rr], round32(w));
+                        if constexpr (rr + 1 < R) w = mul64(w, step_r);
+                    });
+                    if constexpr (qq + 1 < BF) wq = mul64(wq, step_q);
+                });
+            }
+        } else {
+            static_for<0, BF>([&](auto q) {
+                static_for<0, R>([&](auto r) {
+                    constexpr int qq = decltype(q)::value, rr = decltype(r)::value;
+                    v[qq * R + rr] = L[lds_pad(j + T * qq + rr * STRIDE)];
+                });
+            });
+        }
+
+        static_for<0, BF>([&](auto q) {
+            constexpr int qq = decltype(q)::value;
+            if constexpr (s > 0) {
+                const uint32_t k = (j + T * qq) & (NS - 1);
+                static_for<1, R>([&](auto r) {
+                    constexpr int rr = decltype(r)::value;
+                    v[qq * R + rr] = mul(v[qq * R + rr], a.tw[TWO + (rr - 1) * NS + k]);
+                });
+            }
+            dft<R>(&v[qq * R]);
+        });
+
+        if constexpr (s == S - 1) {
+            static_for<0, BF>([&](auto q) {
+                static_for<0, R>([&](auto m) {
+                    constexpr int qq = decltype(q)::value, mm = decltype(m)::value;
+                    dst[out_base + (j + T * qq + mm * STRIDE) * out_step] = maybe_swap<SWAP_OUT>(v[qq * R + mm]);
+                });
+            });
+        } else {
+            if constexpr (s > 0) __syncthreads();
+            static_for<0, BF>([&](auto q) {
+                constexpr int qq = decltype(q)::value;
+                const uint32_t b = j + T * qq;
+                const uint32_t base = ((b >> LNS) << (LNS + LR)) + (b & (NS - 1));
+                static_for<0, R>([&](auto m) {
+                    constexpr int mm = decltype(m)::value;
+                    L[lds_pad(base + mm * NS)] = v[qq * R + mm];
+                });
+            });
+            __syncthreads();
+        }
+    });
+}
+
+}  // namespace onefft

@@ -17,130 +17,18 @@
 // tables (w_n^x = w_n^(x_hi 2^b) w_n^(x_lo)) and a short FP64 recurrence per thread, then rounded
 // once to FP32: the twiddle values are those of the reference's FP64-computed, once-rounded tables
 // (src/onefft.c:112-141) without reading an n-entry table from HBM.
-#include "fft_common.hpp"
-#include "kernels.hpp"
+#include "fft_bodies.hpp"
 
 namespace onefft {
 
 template <int LOG2NF, bool FIRST, bool SWAP_IN, bool SWAP_OUT>
 __global__ __launch_bounds__(pass_wg(LOG2NF)) void fft_pass_kernel(PassArgs a) {
-    constexpr int LOG2P = pass_log2p(LOG2NF);
-    constexpr int NF = 1 << LOG2NF;
-    constexpr int P = 1 << LOG2P;
-    constexpr uint32_t T = NF / P;  // threads per column
-    constexpr uint32_t C = pass_cols(LOG2NF);
-    constexpr int S = stage_count(LOG2NF, LOG2P);
-    constexpr uint32_t SPAN = lds_span(NF) | 1;  // odd: the C columns start on different banks
-    __shared__ cf lds[S > 1 ? C * SPAN : 1];
-
-    const uint32_t t = threadIdx.x;
-    const uint32_t c = t % C;
-    const uint32_t j = t / C;
-    const uint32_t col = blockIdx.x * C + c;
-    const cf *src = a.in + (uint64_t)blockIdx.y * a.in_dist;
-    cf *dst = a.out + (uint64_t)blockIdx.y * a.out_dist;
-    cf *L = lds + (S > 1 ? c * SPAN : 0);
-
-    uint32_t in_base, in_step, out_base, out_step, low = 0;
-    if constexpr (FIRST) {
-        in_base = col;
-        in_step = a.columns;
-        uint32_t row = 0, rest = col;
-        for (uint32_t d = 0; d < a.ndig; d++) {
-            row += (rest & ((1u << a.dig_log2[d]) - 1)) << a.dig_shift[d];
-            rest >>= a.dig_log2[d];
-        }
-        out_base = row << LOG2NF;
-        out_step = 1;
-    } else {
-        low = col & ((1u << a.log2m) - 1);
-        in_base = ((col >> a.log2m) << (a.log2m + LOG2NF)) + low;
-        in_step = 1u << a.log2m;
-        out_base = in_base;
-        out_step = in_step;
-    }
-
-    cf v[P];
-    static_for<0, S>([&](auto sI) {
-        constexpr int s = decltype(sI)::value;
-        constexpr int LR = stage_log2r(LOG2NF, LOG2P, s);
-        constexpr int R = 1 << LR;
-        constexpr int LNS = stage_log2ns(LOG2NF, LOG2P, s);
-        constexpr uint32_t NS = 1u << LNS;
-        constexpr int BF = P / R;
-        constexpr uint32_t STRIDE = NF / R;
-        constexpr uint32_t TWO = stage_tw_offset(LOG2NF, LOG2P, s);
-
-        if constexpr (s == 0) {
-            static_for<0, BF>([&](auto q) {
-                static_for<0, R>([&](auto r) {
-                    constexpr int qq = decltype(q)::value, rr = decltype(r)::value;
-                    v[qq * R + rr] = maybe_swap<SWAP_IN>(src[in_base + (j + T * qq + rr * STRIDE) * in_step]);
-                });
-            });
-            if constexpr (!FIRST) {
-                // element e = j + T q + STRIDE r gets w_n^((e low) << tw_shift)
-                const uint32_t lo_mask = (1u << a.lo_bits) - 1;
-                auto W = [&](uint32_t x) {
-                    x <<= a.tw_shift;
-                    return mul64(a.tw_hi[x >> a.lo_bits], a.tw_lo[x & lo_mask]);
-                };
-                cd wq = W(j * low);
-                const cd step_q = W(T * low);
-                const cd step_r = W(STRIDE * low);
-                static_for<0, BF>([&](auto q) {
-                    constexpr int qq = decltype(q)::value;
-                    cd w = wq;
-                    static_for<0, R>([&](auto r) {
-                        constexpr int rr = decltype(r)::value;
-                        v[qq * R + rr] = mul(v[qq * R + rr], round32(w));
-                        if constexpr (rr + 1 < R) w = mul64(w, step_r);
-                    });
-                    if constexpr (qq + 1 < BF) wq = mul64(wq, step_q);
-                });
-            }
-        } else {
-            static_for<0, BF>([&](auto q) {
-                static_for<0, R>([&](auto r) {
-                    constexpr int qq = decltype(q)::value, rr = decltype(r)::value;
-                    v[qq * R + rr] = L[lds_pad(j + T * qq + rr * STRIDE)];
-                });
-            });
-        }
-
-        static_for<0, BF>([&](auto q) {
-            constexpr int qq = decltype(q)::value;
-            if constexpr (s > 0) {
-                const uint32_t k = (j + T * qq) & (NS - 1);
-                static_for<1, R>([&](auto r) {
-                    constexpr int rr = decltype(r)::value;
-                    v[qq * R + rr] = mul(v[qq * R + rr], a.tw[TWO + (rr - 1) * NS + k]);
-                });
-            }
-            dft<R>(&v[qq * R]);
-        });
-
-        if constexpr (s == S - 1) {
-            static_for<0, BF>([&](auto q) {
-                static_for<0, R>([&](auto m) {
-                    constexpr int qq = decltype(q)::value, mm = decltype(m)::value;
-                    dst[out_base + (j + T * qq + mm * STRIDE) * out_step] = maybe_swap<SWAP_OUT>(v[qq * R + mm]);
-                });
-            });
-        } else {
-            if constexpr (s > 0) __syncthreads();
-            static_for<0, BF>([&](auto q) {
-                constexpr int qq = decltype(q)::value;
-                const uint32_t b = j + T * qq;
-                const uint32_t base = ((b >> LNS) << (LNS + LR)) + (b & (NS - 1));
-                static_for<0, R>([&](auto m) {
-                    constexpr int mm = decltype(m)::value;
-                    L[lds_pad(base + mm * NS)] = v[qq * R + mm];
-                });
-            });
-            __syncthreads();
-        }
-    });
+    constexpr int C = pass_cols(LOG2NF);
+    constexpr bool USES_LDS = stage_count(LOG2NF, pass_log2p(LOG2NF)) > 1;
+    __shared__ cf lds[USES_LDS ? C * pass_span(LOG2NF) : 1];
+    pass_body<LOG2NF, C, FIRST, SWAP_IN, SWAP_OUT>(a, a.in + (uint64_t)blockIdx.y * a.in_dist,
+                                                   a.out + (uint64_t)blockIdx.y * a.out_dist, blockIdx.x, lds,
+                                                   threadIdx.x);
 }
 
 template <int LOG2NF>

@@ -1,0 +1,270 @@
+// Ragged batches in one launch: a persistent kernel whose workgroups claim tasks from a device
+// queue until it is empty.
+//
+// The reference transforms one array per call (include/onefft.h:61-77); a caller with many
+// arrays of mixed sizes calls it once per array. Here every array of a batch is an item
+// (offset, log2 n, direction), and the host orders the tasks so that no wait can deadlock:
+//
+//   1. first-pass tiles of the two-pass items (2^13..2^20 points), largest items first;
+//   2. single-pass tasks (up to 4096 / n items of one size n <= 2^12 per task);
+//   3. second-pass tiles, in the same item order as 1.
+//
+// A second-pass tile waits until every first-pass tile of its item has finished. Tasks are
+// claimed in queue order by running workgroups, so every first-pass tile it waits for has already
+// been claimed by a running workgroup that waits for nothing: the wait always ends. It is bounded
+// anyway; a wait that gives up sets `status` (the host reports it as an error).
+#include "fft_bodies.hpp"
+
+namespace onefft {
+
+namespace {
+
+// LDS of one task: the largest of the single-pass layouts (4096 points) and the pass layouts
+// (C = 4096 / N_f columns of pass_span(N_f) values).
+constexpr uint32_t ragged_lds_values() {
+    uint32_t m = kRaggedTaskPoints / 16 * lds_span(16);
+    for (int l = 6; l <= 10; l++) {
+        uint32_t v = (kRaggedTaskPoints >> l) * pass_span(l);
+        if (v > m) m = v;
+    }
+    return m;
+}
+constexpr uint32_t kLds = ragged_lds_values();
+constexpr uint32_t kSpinLimit = 1u << 25;
+
+// One column transform of N = 2^L points (L <= 12) with every stage's shape chosen at run time:
+// a first stage of radix R0 (N itself below 16 points, else 2^(L mod 4), or 16), then radix-16
+// stages through LDS. The kernel thus has a few short code paths instead of one fully unrolled
+// body per size, which keeps its register use near that of the standalone kernels.
+//
+// Element e of the column is src[e * in_step], output k goes to dst[k * out_step]. With `twiddle`
+// the loaded element e is multiplied by w_n^((e low) << shift), formed in FP64 (second passes).
+struct Column {
+    const cf *src;
+    cf *dst;
+    uint32_t in_step, out_step;
+    bool live, swap_in, swap_out, twiddle;
+    const cd *lo, *hi;
+    uint32_t lo_bits, low, shift;
+};
+
+__device__ __forceinline__ cf swap_if(cf v, bool s) { return s ? swap_ri(v) : v; }
+
+template <int R0>
+__device__ __forceinline__ void first_stage(const Column &io, uint32_t N, uint32_t P, uint32_t j, bool only, cf *L,
+                                            cf (&v)[16]) {
+    const uint32_t T = N / P, stride = N / R0;
+    const uint32_t BF = P / R0;  // 1..8, a run-time count: loops stay short
+    for (uint32_t q = 0; q < BF; q++) {
+        cf w[R0];
+        if (io.live) {
+            static_for<0, R0>([&](auto r) {
+                w[r] = swap_if(io.src[(j + T * q + decltype(r)::value * stride) * io.in_step], io.swap_in);
+            });
+        } else {
+            static_for<0, R0>([&](auto r) { w[r] = mk(0.f, 0.f); });
+        }
+        if (io.twiddle) {
+            const uint32_t mask = (1u << io.lo_bits) - 1;
+            auto W = [&](uint32_t x) {
+                x <<= io.shift;
+                return mul64(io.hi[x >> io.lo_bits], io.lo[x & mask]);
+            };
+            cd tw = W((j + T * q) * io.low);
+            const cd step = W(stride * io.low);
+            static_for<0, R0>([&](auto r) {
+                w[r] = mul(w[r], round32(tw));
+                if constexpr (decltype(r)::value + 1 < R0) tw = mul64(tw, step);
+            });
+        }
+        dft<R0>(w);
+        const uint32_t b = j + T * q;
+        static_for<0, R0>([&](auto m) {
+            constexpr int mm = decltype(m)::value;
+            if (only) {
+                if (io.live) io.dst[(b + mm * stride) * io.out_step] = swap_if(w[mm], io.swap_out);
+            } else {
+                L[lds_pad(b * R0 + mm)] = w[mm];
+            }
+        });
+    }
+    (void)v;
+}
+
+__device__ void column_fft(const Column &io, uint32_t log2n, const cf *tw, cf *L, uint32_t j) {
+    const uint32_t N = 1u << log2n;
+    const uint32_t P = N < 16 ? N : 16;
+    const uint32_t r0_log2 = log2n <= 4 ? log2n : (log2n % 4 ? log2n % 4 : 4);
+    const bool only = log2n <= 4;
+    cf v[16];
+    switch (r0_log2) {
+    case 0: first_stage<1>(io, N, P, j, only, L, v); break;
+    case 1: first_stage<2>(io, N, P, j, only, L, v); break;
+    case 2: first_stage<4>(io, N, P, j, only, L, v); break;
+    case 3: first_stage<8>(io, N, P, j, only, L, v); break;
+    default: first_stage<16>(io, N, P, j, only, L, v); break;
+    }
+    if (only) return;
+    __syncthreads();
+    // radix-16 stages: butterfly j reads j + r N/16, twiddles w_{16 NS}^(r (j mod NS))
+    const uint32_t stride = N / 16;
+    uint32_t log2ns = r0_log2, off = 0;
+    for (;;) {
+        const uint32_t NS = 1u << log2ns;
+        const bool last = log2ns + 4 == log2n;
+        static_for<0, 16>([&](auto r) { v[r] = L[lds_pad(j + decltype(r)::value * stride)]; });
+        const uint32_t k = j & (NS - 1);
+        static_for<1, 16>([&](auto r) {
+            constexpr int rr = decltype(r)::value;
+            v[rr] = mul(v[rr], tw[off + (rr - 1) * NS + k]);
+        });
+        dft<16>(v);
+        if (last) {
+            if (io.live) static_for<0, 16>([&](auto m) {
+                io.dst[(j + decltype(m)::value * stride) * io.out_step] = swap_if(v[m], io.swap_out);
+            });
+            break;
+        }
+        __syncthreads();
+        const uint32_t base = ((j >> log2ns) << (log2ns + 4)) + k;
+        static_for<0, 16>([&](auto m) { L[lds_pad(base + decltype(m)::value * NS)] = v[m]; });
+        __syncthreads();
+        off += 15 * NS;
+        log2ns += 4;
+    }
+}
+
+// up to 4096 / n whole transforms of one size n <= 2^12, one column per transform
+__device__ void single_task(const RaggedArgs &a, const RaggedTask &tk, uint32_t log2n, bool inv, cf *lds,
+                            uint32_t t) {
+    const uint32_t N = 1u << log2n;
+    const uint32_t T = N < 16 ? 1 : N / 16;  // threads per transform
+    const uint32_t c = t / T;
+    Column io;
+    io.live = c < tk.arg;
+    const uint64_t off = io.live ? a.item_off[tk.item + c] : 0;
+    io.src = a.in + off;
+    io.dst = a.out + off;
+    io.in_step = io.out_step = 1;
+    io.swap_in = io.swap_out = inv;
+    io.twiddle = false;
+    column_fft(io, log2n, a.tw32 + tk.tw, lds + c * lds_span(N), t % T);
+}
+
+// one tile (C = 4096 / N_f columns) of one pass of a two-pass transform (index scheme of k_pass.hip)
+__device__ void two_pass_task(const RaggedArgs &a, const RaggedTask &tk, uint32_t kind, uint32_t log2n, bool inv,
+                              cf *lds, uint32_t t) {
+    const RaggedSize &z = a.size[log2n];
+    const uint64_t off = a.item_off[tk.item];
+    const uint32_t tiles = 1u << (log2n - 12);  // tiles per pass
+    const bool first = kind == kRaggedPass1;
+    const uint32_t lf = first ? z.l2 : z.l1;     // this pass's sub-transform
+    const uint32_t C = kRaggedTaskPoints >> lf;  // columns per tile
+    const uint32_t c = t % C, j = t / C;
+    const uint32_t col = tk.arg * C + c;
+    Column io;
+    io.live = true;
+    io.twiddle = !first;
+    io.lo = a.tw64 + z.lo;
+    io.hi = a.tw64 + z.hi;
+    io.lo_bits = z.lo_bits;
+    io.shift = 0;
+    if (first) {
+        // factor 2 (N_2 points) first: column j_1 of the input (stride N_1) -> row at N_2 j_1
+        io.src = a.in + off + col;
+        io.in_step = 1u << z.l1;
+        io.dst = a.out + off + ((uint64_t)col << z.l2);
+        io.out_step = 1;
+        io.swap_in = inv;
+        io.swap_out = false;
+        io.low = 0;
+    } else {
+        if (t == 0) {
+            uint32_t spins = 0;
+            while (__hip_atomic_load(&a.counters[tk.item], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < tiles) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins >= kSpinLimit) {
+                    __hip_atomic_fetch_or(a.status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        // factor 1 (N_1 points) in place: column k_2 at stride N_2, twiddles w_n^(j_1 k_2)
+        io.src = a.out + off + col;
+        io.dst = a.out + off + col;
+        io.in_step = io.out_step = 1u << z.l2;
+        io.swap_in = false;
+        io.swap_out = inv;
+        io.low = col;
+    }
+    column_fft(io, lf, a.tw32 + tk.tw, lds + c * pass_span((int)lf), j);
+    if (first) {
+        // publish: every thread's stores are visible device-wide before the count goes up
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (t == 0) __hip_atomic_fetch_add(&a.counters[tk.item], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(kRaggedWG) void fft_ragged_kernel(RaggedArgs a) {
+    __shared__ cf lds[kLds];
+    __shared__ uint32_t claimed;
+    const uint32_t t = threadIdx.x;
+    for (;;) {
+        if (t == 0) claimed = __hip_atomic_fetch_add(a.head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const uint32_t id = __builtin_amdgcn_readfirstlane(claimed);  // uniform: task fields go to SGPRs
+        __syncthreads();  // everyone has read `claimed` before it is overwritten
+        if (id >= a.ntasks) break;
+        const RaggedTask tk = a.tasks[id];
+        const uint32_t kind = tk.word & 3u, log2n = (tk.word >> 2) & 31u;
+        const bool inv = (tk.word >> 7) & 1u;
+        if (kind == kRaggedSingle) {
+            single_task(a, tk, log2n, inv, lds, t);
+        } else {
+            two_pass_task(a, tk, kind, log2n, inv, lds, t);
+        }
+        __syncthreads();  // LDS is free for the next task
+    }
+}
+
+__global__ __launch_bounds__(256) void fft_ragged_reset(uint32_t *counters, uint32_t count, uint32_t *head,
+                                                        uint32_t *status) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        *head = 0;
+        *status = 0;
+    }
+    for (uint32_t k = i; k < count; k += gridDim.x * blockDim.x) counters[k] = 0;
+}
+
+uint32_t ragged_resident_grid() {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fft_ragged_kernel, kRaggedWG, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return 1024;
+    }
+    if (per < 1) per = 1;
+    return (uint32_t)(cus * per);
+}
+
+hipError_t launch_ragged(const RaggedArgs &a, uint32_t grid, hipStream_t stream) {
+    const uint32_t reset_blocks = a.nitems / 256 + 1 < 1024 ? a.nitems / 256 + 1 : 1024;
+    hipLaunchKernelGGL(fft_ragged_reset, dim3(reset_blocks), dim3(256), 0, stream, a.counters, a.nitems, a.head,
+                       a.status);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (a.ntasks == 0) return hipSuccess;
+    if (grid == 0) grid = ragged_resident_grid();
+    if (grid > a.ntasks) grid = a.ntasks;
+    hipLaunchKernelGGL(fft_ragged_kernel, dim3(grid), dim3(kRaggedWG), 0, stream, a);
+    return hipGetLastError();
+}
+
+}  // namespace onefft

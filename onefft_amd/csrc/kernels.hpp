@@ -71,4 +71,53 @@ struct PassArgs {
 hipError_t launch_pass(int log2nf, bool first, bool swap_in, bool swap_out, const PassArgs &a, uint64_t batch,
                        hipStream_t stream);
 
+// ---- ragged batches: one persistent launch over items of mixed sizes -------------------------
+//
+// Every task is 4096 points of work for a 256-thread workgroup: either up to 4096 / n whole
+// transforms of one size n <= 2^12 (single-pass), or one tile of one pass of a two-pass transform
+// (2^13..2^20 points), C = 4096 / N_f columns of an N_f-point sub-transform.
+constexpr int kRaggedMaxLog2 = 20;
+constexpr int kRaggedSingleMaxLog2 = 12;
+constexpr int kRaggedWG = 256;
+constexpr uint32_t kRaggedTaskPoints = 4096;
+
+enum RaggedKind : uint32_t { kRaggedSingle = 0, kRaggedPass1 = 1, kRaggedPass2 = 2 };
+
+struct RaggedTask {
+    uint32_t word;  // kind | log2n << 2 | inverse << 7
+    uint32_t item;  // single: first index into the sorted item list; pass: the item's index
+    uint32_t arg;   // single: number of transforms; pass: tile index
+    uint32_t tw;    // offset of the stage table this task reads, in the FP32 table blob (read per
+                    // task, so the compiler cannot hoist twiddle loads out of the task loop)
+};
+
+struct RaggedSize {
+    uint32_t tw;       // single-pass sizes: offset of the stage table in the FP32 table blob
+    uint32_t l1, l2;   // two-pass sizes: log2 N_1 (transformed last) and log2 N_2
+    uint32_t tw1, tw2; // stage tables of N_1 and N_2
+    uint32_t lo, hi;   // offsets of the FP64 twiddle tables of w_n in the FP64 blob
+    uint32_t lo_bits;
+};
+
+struct RaggedArgs {
+    const cf *in;
+    cf *out;
+    const RaggedTask *tasks;
+    const uint64_t *item_off;  // complex offset of each item, in task order
+    uint32_t *counters;        // finished first-pass tiles per item
+    uint32_t *head;            // next task to claim
+    uint32_t *status;          // nonzero: a wait gave up (a bug; the result is wrong)
+    const cf *tw32;
+    const cd *tw64;
+    uint32_t ntasks;
+    uint32_t nitems;
+    RaggedSize size[kRaggedMaxLog2 + 1];
+};
+
+// Zeroes the queue head, status and item counters, then runs the persistent kernel on `grid`
+// workgroups (0: as many as fit the device at once).
+hipError_t launch_ragged(const RaggedArgs &a, uint32_t grid, hipStream_t stream);
+// Workgroups of the persistent kernel that fit the device at once.
+uint32_t ragged_resident_grid();
+
 }  // namespace onefft

@@ -1,0 +1,291 @@
+// Ragged-batch plans: the host builds the task queue and the tables once; an execution is one
+// reset kernel plus one persistent kernel (k_ragged.hip).
+//
+// No counterpart in the reference, whose API transforms one array per call
+// (include/onefft.h:61-77); onefft_ragged_execute_per_item runs the same items that way (one
+// plan call per item on the device) for comparison.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/onefft.h"
+#include "fft_common.hpp"
+#include "kernels.hpp"
+#include "runtime.hpp"
+
+using onefft::cd;
+using onefft::cf;
+
+struct onefft_ragged_plan {
+    int device = 0;
+    size_t count = 0;
+    uint64_t extent = 0;  // complex values the buffers must hold
+    void *dmem = nullptr;
+    onefft::RaggedArgs args{};
+    uint32_t grid = 0;
+    std::vector<onefft_ragged_item> items;  // as given, for the per-item comparison
+};
+
+namespace onefft {
+namespace {
+
+uint32_t per_task(unsigned log2n) {
+    const unsigned lp = (unsigned)batch_log2p((int)log2n);
+    return (uint32_t)kRaggedWG >> (log2n - lp);
+}
+
+}  // namespace
+}  // namespace onefft
+
+using namespace onefft;
+
+extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_item *items, size_t count) {
+    if (!items && count) {
+        set_error("null item list");
+        return nullptr;
+    }
+    if (count > 0xffffffffu) {
+        set_error("too many items");
+        return nullptr;
+    }
+    // validate: sizes, directions, and no two items overlap
+    std::vector<size_t> by_offset(count);
+    for (size_t i = 0; i < count; i++) {
+        if (items[i].log2n > (uint32_t)kRaggedMaxLog2 || items[i].inverse > 1) {
+            set_error("ragged item size above 2^20 or direction not 0/1");
+            return nullptr;
+        }
+        by_offset[i] = i;
+    }
+    std::sort(by_offset.begin(), by_offset.end(),
+              [&](size_t a, size_t b) { return items[a].offset < items[b].offset; });
+    uint64_t extent = 0;
+    for (size_t k = 0; k < count; k++) {
+        const onefft_ragged_item &it = items[by_offset[k]];
+        if (it.offset < extent) {
+            set_error("ragged items overlap");
+            return nullptr;
+        }
+        extent = it.offset + (uint64_t(1) << it.log2n);
+    }
+
+    onefft_ragged_plan *plan = new (std::nothrow) onefft_ragged_plan;
+    if (!plan) return nullptr;
+    if (hipGetDevice(&plan->device) != hipSuccess) {
+        set_error("no HIP device");
+        delete plan;
+        return nullptr;
+    }
+    plan->count = count;
+    plan->extent = extent;
+    plan->items.assign(items, items + count);
+
+    // tables: one FP32 stage table per size used as a single pass or as a sub-transform, and the
+    // two-level FP64 table of w_n per two-pass size
+    bool used[kRaggedMaxLog2 + 1] = {};
+    for (size_t i = 0; i < count; i++) used[items[i].log2n] = true;
+    std::vector<cf> tw32(1);
+    std::vector<cd> tw64(1);
+    int32_t stage_at[kRaggedMaxLog2 + 1];
+    for (auto &v : stage_at) v = -1;
+    auto stage_table = [&](int log2) -> uint32_t {
+        if (stage_at[log2] < 0) {
+            const int lp = batch_log2p(log2);
+            const uint32_t entries = stage_tw_size(log2, lp);
+            stage_at[log2] = (int32_t)tw32.size();
+            tw32.resize(tw32.size() + entries + 1);
+            if (entries) fill_stage_table(log2, lp, tw32.data() + stage_at[log2]);
+        }
+        return (uint32_t)stage_at[log2];
+    };
+    RaggedArgs &a = plan->args;
+    std::memset(&a, 0, sizeof(a));
+    for (int l = 0; l <= kRaggedMaxLog2; l++) {
+        if (!used[l]) continue;
+        RaggedSize &z = a.size[l];
+        if (l <= kRaggedSingleMaxLog2) {
+            z.tw = stage_table(l);
+        } else {
+            z.l1 = (uint32_t)l / 2;
+            z.l2 = (uint32_t)l - z.l1;
+            z.tw1 = stage_table((int)z.l1);
+            z.tw2 = stage_table((int)z.l2);
+            z.lo_bits = ((uint32_t)l + 1) / 2;
+            z.lo = (uint32_t)tw64.size();
+            for (uint64_t i = 0; i < (uint64_t(1) << z.lo_bits); i++) {
+                cd w;
+                root64(i, (unsigned)l, &w.r, &w.i);
+                tw64.push_back(w);
+            }
+            z.hi = (uint32_t)tw64.size();
+            for (uint64_t i = 0; i < (uint64_t(1) << (l - z.lo_bits)); i++) {
+                cd w;
+                root64(i << z.lo_bits, (unsigned)l, &w.r, &w.i);
+                tw64.push_back(w);
+            }
+        }
+    }
+
+    // items in task order: two-pass items (largest first), then single-pass items grouped by size
+    // (largest first) and direction
+    std::vector<uint32_t> order(count);
+    for (size_t i = 0; i < count; i++) order[i] = (uint32_t)i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+        if (items[x].log2n != items[y].log2n) return items[x].log2n > items[y].log2n;
+        return items[x].inverse < items[y].inverse;
+    });
+    size_t nbig = 0;
+    while (nbig < count && items[order[nbig]].log2n > (uint32_t)kRaggedSingleMaxLog2) nbig++;
+    std::vector<uint64_t> item_off(count ? count : 1);
+    for (size_t k = 0; k < count; k++) item_off[k] = items[order[k]].offset;
+
+    std::vector<RaggedTask> tasks;
+    auto word = [](uint32_t kind, uint32_t log2n, uint32_t inv) { return kind | (log2n << 2) | (inv << 7); };
+    for (size_t k = 0; k < nbig; k++) {
+        const onefft_ragged_item &it = items[order[k]];
+        for (uint32_t tile = 0; tile < (1u << (it.log2n - 12)); tile++)
+            tasks.push_back({word(kRaggedPass1, it.log2n, it.inverse), (uint32_t)k, tile, a.size[it.log2n].tw2});
+    }
+    for (size_t k = nbig; k < count;) {
+        const onefft_ragged_item &it = items[order[k]];
+        size_t e = k;
+        const uint32_t cap = per_task(it.log2n);
+        while (e < count && e - k < cap && items[order[e]].log2n == it.log2n && items[order[e]].inverse == it.inverse)
+            e++;
+        tasks.push_back({word(kRaggedSingle, it.log2n, it.inverse), (uint32_t)k, (uint32_t)(e - k), a.size[it.log2n].tw});
+        k = e;
+    }
+    for (size_t k = 0; k < nbig; k++) {
+        const onefft_ragged_item &it = items[order[k]];
+        for (uint32_t tile = 0; tile < (1u << (it.log2n - 12)); tile++)
+            tasks.push_back({word(kRaggedPass2, it.log2n, it.inverse), (uint32_t)k, tile, a.size[it.log2n].tw1});
+    }
+    if (tasks.size() > 0xffffffffu) {
+        set_error("too many tasks");
+        delete plan;
+        return nullptr;
+    }
+
+    // one device allocation: tasks | item offsets | counters | head, status | tw32 | tw64
+    auto up = [](size_t v) { return (v + 255) / 256 * 256; };
+    const size_t o_tasks = 0;
+    const size_t o_off = up(o_tasks + tasks.size() * sizeof(RaggedTask));
+    const size_t o_cnt = up(o_off + item_off.size() * sizeof(uint64_t));
+    const size_t o_head = up(o_cnt + (nbig ? nbig : 1) * sizeof(uint32_t));
+    const size_t o_tw32 = up(o_head + 2 * sizeof(uint32_t));
+    const size_t o_tw64 = up(o_tw32 + tw32.size() * sizeof(cf));
+    const size_t bytes = up(o_tw64 + tw64.size() * sizeof(cd));
+    std::vector<unsigned char> image(bytes, 0);
+    std::memcpy(image.data() + o_tasks, tasks.data(), tasks.size() * sizeof(RaggedTask));
+    std::memcpy(image.data() + o_off, item_off.data(), item_off.size() * sizeof(uint64_t));
+    std::memcpy(image.data() + o_tw32, tw32.data(), tw32.size() * sizeof(cf));
+    std::memcpy(image.data() + o_tw64, tw64.data(), tw64.size() * sizeof(cd));
+    if (hipMalloc(&plan->dmem, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("ragged plan allocation failed");
+        delete plan;
+        return nullptr;
+    }
+    if (hipMemcpy(plan->dmem, image.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(plan->dmem);
+        set_error("ragged plan upload failed");
+        delete plan;
+        return nullptr;
+    }
+    unsigned char *d = static_cast<unsigned char *>(plan->dmem);
+    a.tasks = reinterpret_cast<const RaggedTask *>(d + o_tasks);
+    a.item_off = reinterpret_cast<const uint64_t *>(d + o_off);
+    a.counters = reinterpret_cast<uint32_t *>(d + o_cnt);
+    a.head = reinterpret_cast<uint32_t *>(d + o_head);
+    a.status = a.head + 1;
+    a.tw32 = reinterpret_cast<const cf *>(d + o_tw32);
+    a.tw64 = reinterpret_cast<const cd *>(d + o_tw64);
+    a.ntasks = (uint32_t)tasks.size();
+    a.nitems = (uint32_t)nbig;
+    plan->grid = ragged_resident_grid();
+    // the per-item comparison uses the cached plan of every size
+    for (int l = 0; l <= kRaggedMaxLog2; l++)
+        if (used[l] && !cached_plan((size_t)1 << l)) {
+            (void)hipFree(plan->dmem);
+            delete plan;
+            return nullptr;
+        }
+    return plan;
+}
+
+extern "C" void onefft_ragged_plan_destroy(onefft_ragged_plan *plan) {
+    if (!plan) return;
+    if (plan->dmem) (void)hipFree(plan->dmem);
+    delete plan;
+}
+
+extern "C" size_t onefft_ragged_extent(const onefft_ragged_plan *plan) { return plan ? (size_t)plan->extent : 0; }
+
+extern "C" size_t onefft_ragged_task_count(const onefft_ragged_plan *plan) { return plan ? plan->args.ntasks : 0; }
+
+static bool ragged_args_ok(const onefft_ragged_plan *plan, const float *d_in, float *d_out) {
+    if (!plan || (plan->count && (!d_in || !d_out))) {
+        set_error("null plan or buffer");
+        return false;
+    }
+    if (((uintptr_t)d_in | (uintptr_t)d_out) & 7) {
+        set_error("device buffers must be 8-byte aligned");
+        return false;
+    }
+    return true;
+}
+
+extern "C" int onefft_ragged_execute(const onefft_ragged_plan *plan, const float *d_in, float *d_out,
+                                     struct ihipStream_t *stream) {
+    if (!ragged_args_ok(plan, d_in, d_out)) return -1;
+    RaggedArgs a = plan->args;
+    a.in = reinterpret_cast<const cf *>(d_in);
+    a.out = reinterpret_cast<cf *>(d_out);
+    hipError_t e = launch_ragged(a, plan->grid, stream);
+    if (e != hipSuccess) {
+        set_error("ragged launch", e);
+        return -1;
+    }
+    return 0;
+}
+
+extern "C" int onefft_ragged_execute_per_item(const onefft_ragged_plan *plan, const float *d_in, float *d_out,
+                                              struct ihipStream_t *stream) {
+    if (!ragged_args_ok(plan, d_in, d_out)) return -1;
+    const cf *in = reinterpret_cast<const cf *>(d_in);
+    cf *out = reinterpret_cast<cf *>(d_out);
+    for (const onefft_ragged_item &it : plan->items) {
+        const size_t n = (size_t)1 << it.log2n;
+        const onefft_plan *p = cached_plan(n);
+        if (!p || execute_device(p, in + it.offset, out + it.offset, 1, n, n, it.inverse != 0, stream) != 0)
+            return -1;
+    }
+    return 0;
+}
+
+extern "C" int onefft_ragged_wait(const onefft_ragged_plan *plan, struct ihipStream_t *stream) {
+    if (!plan) {
+        set_error("null plan");
+        return -1;
+    }
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) {
+        set_error("ragged execution", e);
+        return -1;
+    }
+    uint32_t status = 0;
+    e = hipMemcpy(&status, plan->args.status, sizeof(status), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        set_error("ragged status read", e);
+        return -1;
+    }
+    if (status) {
+        set_error("ragged execution: a second-pass wait gave up");
+        return -1;
+    }
+    return 0;
+}

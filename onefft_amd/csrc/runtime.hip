@@ -24,8 +24,10 @@ using onefft::cd;
 using onefft::cf;
 
 namespace {
-
 constexpr unsigned kMaxLog2 = 30;  // the reference's limit (src/onefft.c:1117-1121)
+}  // namespace
+
+namespace onefft {
 
 // ---- twiddles -------------------------------------------------------------------------------
 //
@@ -79,6 +81,13 @@ void fill_stage_table(int log2n, int log2p, cf *table) {
             for (uint32_t k = 0; k < ns; k++) table[off + (r - 1) * ns + k] = root32((uint64_t)r * k, lns + lr);
     }
 }
+
+}  // namespace onefft
+
+namespace {
+
+using onefft::root64;
+using onefft::fill_stage_table;
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
@@ -389,7 +398,7 @@ extern "C" int onefft_plan_inverse_batch(const onefft_plan *plan, const float *d
 
 static std::atomic<onefft_plan *> g_cache[64];
 
-static const onefft_plan *cache_get(size_t n) {
+const onefft_plan *onefft::cached_plan(size_t n) {
     unsigned log2n;
     if (!size_log2(n, &log2n)) return nullptr;
     std::atomic<onefft_plan *> &slot = g_cache[log2n];
@@ -406,19 +415,19 @@ static const onefft_plan *cache_get(size_t n) {
 }
 
 extern "C" int onefft_forward(const float *in, float *out, size_t n) {
-    const onefft_plan *plan = cache_get(n);
+    const onefft_plan *plan = cached_plan(n);
     if (!plan) return -1;
     return run_sync(plan, in, out, false);
 }
 
 extern "C" int onefft_inverse(const float *in, float *out, size_t n) {
-    const onefft_plan *plan = cache_get(n);
+    const onefft_plan *plan = cached_plan(n);
     if (!plan) return -1;
     return run_sync(plan, in, out, true);
 }
 
 extern "C" int onefft_cache_prepare(size_t n) {
-    const onefft_plan *plan = cache_get(n);
+    const onefft_plan *plan = cached_plan(n);
     if (!plan) return -1;
     return staging_reserve(plan->device, n) ? 0 : -1;
 }

@@ -5,6 +5,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "fft_common.hpp"
 #include "kernels.hpp"
 
 struct onefft_plan;
@@ -28,6 +29,15 @@ bool staging_reserve(int device, size_t n);
 
 // The synchronous drop-in call: host or device pointers of any alignment.
 int execute_host_call(const onefft_plan *plan, const float *in, float *out, bool inverse);
+
+// exp(-2 pi i k / 2^log2size) in FP64 (octant-folded), and its FP32 rounding.
+void root64(uint64_t k, unsigned log2size, double *re, double *im);
+cf root32(uint64_t k, unsigned log2size);
+// Stage twiddles of an in-LDS transform of 2^log2n points with 2^log2p points per thread.
+void fill_stage_table(int log2n, int log2p, cf *table);
+
+// The cache's plan for size n (created on first use), or nullptr.
+const onefft_plan *cached_plan(size_t n);
 
 // The calling thread's stream on `device` (created on first use), or nullptr on failure.
 hipStream_t thread_stream(int device);

@@ -87,6 +87,8 @@ def _load_ref(name):
         L.ref_batch.argtypes = [_F, _F, c_size_t, c_size_t, c_size_t, c_int, c_int]
         L.ref_time_batch.argtypes = [_F, _F, c_size_t, c_size_t, c_size_t, c_int, c_int, c_double, c_int]
         L.ref_time_batch.restype = c_double
+        L.ref_ragged.argtypes = [_F, _F, c_void_p, c_size_t, c_int, c_int]
+        L.ref_ragged.restype = c_double
     _ref[name] = L
     return L
 
@@ -225,6 +227,29 @@ def ref_time_batch(x: np.ndarray, n: int, batch: int, threads: int, inverse: boo
     x = np.ascontiguousarray(x, np.float32)
     out = np.empty_like(x)
     return L.ref_time_batch(_fp(x), _fp(out), n, batch, n, int(inverse), threads, sample_seconds, samples)
+
+
+def _items_array(items):
+    arr = np.zeros(max(len(items), 1), dtype=[("offset", "<u8"), ("log2n", "<u4"), ("inverse", "<u4")])
+    for i, (off, l, inv) in enumerate(items):
+        arr[i] = (off, l, inv)
+    return arr
+
+
+def ref_ragged(x: np.ndarray, items, threads: int = 1, reps: int = 1, out: np.ndarray | None = None):
+    """The reference's transform of every (offset, log2n, inverse) item of a ragged batch, on
+    `threads` threads sharing its plan cache. Returns (output, seconds of the fastest pass), or
+    None when the reference was not built. Gaps between items keep `out`'s values."""
+    L = ref()
+    if L is None:
+        return None
+    x = np.ascontiguousarray(x, np.float32)
+    if out is None:
+        out = np.zeros_like(x)
+    arr = _items_array(items)
+    secs = L.ref_ragged(_fp(x), _fp(out), arr.ctypes.data, len(items), threads, reps)
+    assert secs >= 0, "reference ragged run failed"
+    return out, secs
 
 
 def ref_implementation() -> str | None:

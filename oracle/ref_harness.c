@@ -145,3 +145,75 @@ int ref_online_cpus(void) {
     int cpus[1024];
     return ref_cpus(cpus, 1024);
 }
+
+/* ---- ragged batches: every item through the reference's cache API ---------------------------- */
+
+typedef struct {
+    uint64_t offset; /* complex values */
+    uint32_t log2n;
+    uint32_t inverse;
+} ref_item;
+
+typedef struct {
+    const float *in;
+    float *out;
+    const ref_item *items;
+    size_t count;
+    size_t *next;
+    int cpu;
+    pthread_barrier_t *start;
+    int failures;
+} ref_ragged_job;
+
+static void *ref_ragged_worker(void *arg) {
+    ref_ragged_job *job = (ref_ragged_job *)arg;
+    ref_pin(job->cpu);
+    pthread_barrier_wait(job->start);
+    for (;;) {
+        size_t i = __atomic_fetch_add(job->next, 1, __ATOMIC_RELAXED);
+        if (i >= job->count) break;
+        const ref_item *it = &job->items[i];
+        const float *x = job->in + 2 * it->offset;
+        float *y = job->out + 2 * it->offset;
+        size_t n = (size_t)1 << it->log2n;
+        int status = it->inverse ? ref_onefft_inverse(x, y, n) : ref_onefft_forward(x, y, n);
+        if (status) job->failures++;
+    }
+    return NULL;
+}
+
+/* Transforms every item on `threads` threads pulling items from an atomic counter, after
+ * ref_onefft_cache_prepare for every size used. Returns the seconds of the fastest of `reps`
+ * passes over the items, or -1 on error. */
+double ref_ragged(const float *in, float *out, const ref_item *items, size_t count, int threads, int reps) {
+    for (size_t i = 0; i < count; i++)
+        if (ref_onefft_cache_prepare((size_t)1 << items[i].log2n)) return -1;
+    int cpus[1024];
+    int ncpu = ref_cpus(cpus, 1024);
+    if (threads > 256) threads = 256;
+    if (threads < 1) threads = 1;
+    double best = 1e300;
+    for (int r = 0; r < (reps < 1 ? 1 : reps); r++) {
+        pthread_t tid[256];
+        ref_ragged_job jobs[256];
+        size_t next = 0;
+        pthread_barrier_t start;
+        pthread_barrier_init(&start, NULL, (unsigned)threads + 1);
+        for (int t = 0; t < threads; t++) {
+            jobs[t] = (ref_ragged_job){in, out, items, count, &next, ncpu >= threads ? cpus[t] : -1, &start, 0};
+            pthread_create(&tid[t], NULL, ref_ragged_worker, &jobs[t]);
+        }
+        pthread_barrier_wait(&start);
+        double t0 = ref_now();
+        int failures = 0;
+        for (int t = 0; t < threads; t++) {
+            pthread_join(tid[t], NULL);
+            failures += jobs[t].failures;
+        }
+        double dt = ref_now() - t0;
+        pthread_barrier_destroy(&start);
+        if (failures) return -1;
+        if (dt < best) best = dt;
+    }
+    return best;
+}

@@ -222,8 +222,9 @@ def test_large_2_27():
     bins = [0, 1, k0, n // 2, n - 1, 12345677]
     want = O.dft_bins_f64(x, n, bins)
     got = np.concatenate([y[2 * b: 2 * b + 2] for b in bins]).astype(np.float64)
-    # bins of an FFT of |x| ~ 1 carry absolute error ~ eps * sqrt(n) log n
-    assert np.abs(got - want).max() < 1e-6 * np.sqrt(n) * L
+    # a bin carries absolute error ~ eps sqrt(n) log n from the noise plus a relative error on
+    # its own magnitude (the tone bin is n)
+    assert (np.abs(got - want) < 1e-6 * np.sqrt(n) * L + tol_fp64(L) * np.abs(want)).all()
     del ref
     back = gpu_batch(y, n, 1, True)
     assert O.rel_rms(back, x.astype(np.float64) * n) < 2 * tol_fp64(L)

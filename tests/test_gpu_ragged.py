@@ -1,0 +1,122 @@
+"""Ragged batches in one persistent launch (onefft_ragged_*), through the C ABI.
+
+Every item is checked against the FP64 oracle (4e-7 + 1e-7 log2 n, the reference tests' bound)
+and, at the workload's size, against the reference's own CPU transform of every item
+(rel-L2 <= 3e-7 log2 n); the space between items is never written; the per-item comparison path
+gives the same results within the same bounds.
+"""
+import numpy as np
+import pytest
+import torch
+
+import onefft_amd as F
+from conftest import tol_fp64, tol_ref
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CANARY = 12345.0
+
+
+def stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def layout(sizes, gap=0, seed=0):
+    rng = np.random.default_rng(seed)
+    items, off = [], 0
+    for l in sizes:
+        items.append((off, int(l), int(rng.integers(0, 2))))
+        off += (1 << int(l)) + gap
+    return items, off
+
+
+def run(items, extent, x, per_item=False):
+    d_in = torch.from_numpy(x).cuda()
+    d_out = torch.full((2 * extent,), CANARY, device="cuda")
+    p = F.RaggedPlan(items)
+    assert p.extent <= extent
+    if per_item:
+        p.execute_per_item(d_in.data_ptr(), d_out.data_ptr(), stream())
+        torch.cuda.synchronize()
+    else:
+        p.execute(d_in.data_ptr(), d_out.data_ptr(), stream())
+        p.wait(stream())
+    assert np.array_equal(d_in.cpu().numpy(), x)
+    return d_out.cpu().numpy(), p
+
+
+def check_items(items, x, y, extent, fp64_max_log2=20):
+    covered = np.zeros(extent, bool)
+    for off, l, inv in items:
+        n = 1 << l
+        seg = slice(2 * off, 2 * (off + n))
+        covered[off:off + n] = True
+        if l <= fp64_max_log2:
+            e = O.rel_rms(y[seg], O.fft_f64(x[seg], n, bool(inv)))
+            assert e < tol_fp64(l), (off, l, inv, e)
+    gaps = np.repeat(~covered, 2)
+    assert (y[gaps] == CANARY).all(), "wrote between items"
+
+
+@pytest.mark.parametrize("per_item", [False, True])
+def test_every_size_with_gaps(per_item):
+    sizes = list(range(0, 21)) + list(range(20, -1, -1)) + [4, 4, 4, 12, 13, 3, 3]
+    items, extent = layout(sizes, gap=3, seed=1)
+    x = O.normal(31, extent)
+    y, p = run(items, extent, x, per_item)
+    check_items(items, x, y, extent)
+
+
+def test_many_small_items_fill_tasks():
+    rng = np.random.default_rng(5)
+    sizes = rng.integers(0, 13, 3000)
+    items, extent = layout(sizes, seed=2)
+    x = O.uniform(32, extent)
+    y, p = run(items, extent, x)
+    check_items(items, x, y, extent)
+
+
+def test_two_pass_items_many_tiles():
+    sizes = [20, 19, 18, 17, 16, 15, 14, 13] * 2
+    items, extent = layout(sizes, gap=1, seed=3)
+    x = O.normal(33, extent)
+    y, p = run(items, extent, x)
+    check_items(items, x, y, extent)
+
+
+def test_rejects_bad_items():
+    lib = F.lib()
+    for items in ([(0, 21, 0)], [(0, 4, 2)], [(0, 4, 0), (15, 4, 0)], [(16, 4, 0), (0, 5, 0)]):
+        with pytest.raises(F.OneFFTError):
+            F.RaggedPlan(items)
+    p = F.RaggedPlan([])
+    assert p.extent == 0 and p.tasks == 0
+    p.execute(0, 0, stream())
+    p.wait(stream())
+    assert lib.onefft_ragged_execute(None, None, None, None) == -1
+
+
+def test_workload_prefix_against_reference():
+    """The ragged workload (seed 3, log2 n uniform on 4..20, directions 50/50), the longest prefix
+    whose input fits 4 GiB, in one launch: every item against the reference CPU transform."""
+    items = F.ragged_items(3, 200000, max_bytes=4 << 30)
+    extent = items[-1][0] + (1 << items[-1][1])
+    x = O.normal(3, extent)
+    print(f"ragged prefix: {len(items)} items, {extent * 8 / 2**30:.2f} GiB", flush=True)
+    y, p = run(items, extent, x)
+    r = O.ref_ragged(x, items, threads=16)
+    if r is None:
+        pytest.skip("reference not built")
+    ref, _ = r
+    worst = 0.0
+    for off, l, inv in items:
+        seg = slice(2 * off, 2 * (off + (1 << l)))
+        e = O.rel_rms(y[seg], ref[seg])
+        worst = max(worst, e / tol_ref(l))
+    assert worst < 1.0, worst
+    # FP64 on a sample of items
+    for k in np.random.default_rng(0).choice(len(items), 40, replace=False):
+        off, l, inv = items[k]
+        seg = slice(2 * off, 2 * (off + (1 << l)))
+        assert O.rel_rms(y[seg], O.fft_f64(x[seg], 1 << l, bool(inv))) < tol_fp64(l)

@@ -20,8 +20,9 @@ namespace onefft {
 //
 // Thread j of the transform (0 <= j < 2^LOG2N / P) works on src/dst; `live` false skips the global
 // loads and stores but keeps the thread in every barrier. L is the transform's LDS region of
-// lds_span(2^LOG2N) values (unused when the transform fits one thread).
-template <int LOG2N, bool INV>
+// lds_extent<LAYOUT>(2^LOG2N) values (unused when the transform fits one thread). NT: nontemporal
+// global loads and stores.
+template <int LOG2N, bool INV, bool NT = false, int LAYOUT = kPadded>
 __device__ __forceinline__ void single_body(const cf *__restrict__ src, cf *__restrict__ dst, bool live,
                                             const cf *__restrict__ tw, cf *L, uint32_t j) {
     constexpr int LOG2P = batch_log2p(LOG2N);
@@ -46,7 +47,7 @@ __device__ __forceinline__ void single_body(const cf *__restrict__ src, cf *__re
                 static_for<0, BF>([&](auto q) {
                     static_for<0, R>([&](auto r) {
                         constexpr int qq = decltype(q)::value, rr = decltype(r)::value;
-                        v[qq * R + rr] = maybe_swap<INV>(src[j + T * qq + rr * STRIDE]);
+                        v[qq * R + rr] = maybe_swap<INV>(gload<NT>(&src[j + T * qq + rr * STRIDE]));
                     });
                 });
             } else {
@@ -56,7 +57,7 @@ __device__ __forceinline__ void single_body(const cf *__restrict__ src, cf *__re
             static_for<0, BF>([&](auto q) {
                 static_for<0, R>([&](auto r) {
                     constexpr int qq = decltype(q)::value, rr = decltype(r)::value;
-                    v[qq * R + rr] = L[lds_pad(j + T * qq + rr * STRIDE)];
+                    v[qq * R + rr] = L[lds_index<LAYOUT>(j + T * qq + rr * STRIDE)];
                 });
             });
         }
@@ -79,7 +80,7 @@ __device__ __forceinline__ void single_body(const cf *__restrict__ src, cf *__re
                 static_for<0, BF>([&](auto q) {
                     static_for<0, R>([&](auto m) {
                         constexpr int qq = decltype(q)::value, mm = decltype(m)::value;
-                        dst[j + T * qq + mm * STRIDE] = maybe_swap<INV>(v[qq * R + mm]);
+                        gstore<NT>(&dst[j + T * qq + mm * STRIDE], maybe_swap<INV>(v[qq * R + mm]));
                     });
                 });
             }
@@ -91,7 +92,7 @@ __device__ __forceinline__ void single_body(const cf *__restrict__ src, cf *__re
                 const uint32_t base = ((b >> LNS) << (LNS + LR)) + (b & (NS - 1));
                 static_for<0, R>([&](auto m) {
                     constexpr int mm = decltype(m)::value;
-                    L[lds_pad(base + mm * NS)] = v[qq * R + mm];
+                    L[lds_index<LAYOUT>(base + mm * NS)] = v[qq * R + mm];
                 });
             });
             __syncthreads();

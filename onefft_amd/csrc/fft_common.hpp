@@ -198,9 +198,50 @@ __host__ __device__ constexpr uint32_t stage_tw_size(int log2n, int log2p) {
 
 // ---- LDS layout ---------------------------------------------------------------------------
 //
-// One complex of padding after every 16: stage accesses that stride by powers of two then
-// spread over the banks.
-__host__ __device__ constexpr uint32_t lds_pad(uint32_t i) { return i + (i >> 4); }
-__host__ __device__ constexpr uint32_t lds_span(uint32_t n) { return n + (n >> 4); }
+// Two layouts of a transform's values in LDS, chosen per kernel variant:
+//   kPadded   one complex of padding after every 16: stage accesses that stride by powers of two
+//             spread over the banks, at 1/16 extra LDS;
+//   kSwizzled no padding; the low 4 index bits are XORed with bits 4..7 and 8..11, a bijection
+//             that spreads the same accesses over the banks and keeps a 4096-point transform at
+//             exactly 32 KiB (one more workgroup per CU).
+enum LdsLayout : int { kPadded = 0, kSwizzled = 1 };
+
+template <int LAYOUT = kPadded>
+__host__ __device__ constexpr uint32_t lds_index(uint32_t i) {
+    if constexpr (LAYOUT == kSwizzled) return i ^ (((i >> 4) ^ (i >> 8)) & 15u);
+    else return i + (i >> 4);
+}
+template <int LAYOUT = kPadded>
+__host__ __device__ constexpr uint32_t lds_extent(uint32_t n) {
+    return LAYOUT == kSwizzled ? n : n + (n >> 4);
+}
+// the padded layout, used by the kernels without a layout choice
+__host__ __device__ constexpr uint32_t lds_pad(uint32_t i) { return lds_index<kPadded>(i); }
+__host__ __device__ constexpr uint32_t lds_span(uint32_t n) { return lds_extent<kPadded>(n); }
+
+// ---- global access --------------------------------------------------------------------------
+//
+// Streamed data (read once, written once) can use nontemporal loads and stores, which keep it
+// from displacing the twiddle tables in the caches.
+typedef float f2v __attribute__((ext_vector_type(2)));
+
+template <bool NT>
+__device__ __forceinline__ cf gload(const cf *p) {
+    if constexpr (NT) {
+        f2v v = __builtin_nontemporal_load(reinterpret_cast<const f2v *>(p));
+        return mk(v.x, v.y);
+    } else {
+        return *p;
+    }
+}
+template <bool NT>
+__device__ __forceinline__ void gstore(cf *p, cf c) {
+    if constexpr (NT) {
+        f2v v = {c.r, c.i};
+        __builtin_nontemporal_store(v, reinterpret_cast<f2v *>(p));
+    } else {
+        *p = c;
+    }
+}
 
 }  // namespace onefft

@@ -11,7 +11,7 @@
 
 namespace onefft {
 
-template <int LOG2N, bool INV>
+template <int LOG2N, bool INV, bool NT, int LAYOUT>
 __global__ __launch_bounds__(batch_wg(LOG2N)) void fft_batch_kernel(const cf *__restrict__ in, cf *__restrict__ out,
                                                                       const cf *__restrict__ tw, uint64_t in_dist,
                                                                       uint64_t out_dist, uint64_t batch) {
@@ -19,50 +19,63 @@ __global__ __launch_bounds__(batch_wg(LOG2N)) void fft_batch_kernel(const cf *__
     constexpr uint32_t T = N >> batch_log2p(LOG2N);  // threads per transform
     constexpr int PER = batch_per_wg(LOG2N);         // transforms per workgroup
     constexpr bool USES_LDS = stage_count(LOG2N, batch_log2p(LOG2N)) > 1;
-    constexpr uint32_t SPAN = lds_span(N);
+    constexpr uint32_t SPAN = lds_extent<LAYOUT>(N);
     static_assert(batch_wg(LOG2N) % T == 0, "workgroup must hold whole transforms");
     __shared__ cf lds[USES_LDS ? PER * SPAN : 1];
 
     const uint32_t t = threadIdx.x;
     const uint32_t c = t / T;
     const uint64_t which = (uint64_t)blockIdx.x * PER + c;
-    single_body<LOG2N, INV>(in + which * in_dist, out + which * out_dist, which < batch, tw,
-                            lds + (USES_LDS ? c * SPAN : 0), t % T);
+    single_body<LOG2N, INV, NT, LAYOUT>(in + which * in_dist, out + which * out_dist, which < batch, tw,
+                                        lds + (USES_LDS ? c * SPAN : 0), t % T);
+}
+
+template <int LOG2N, bool NT, int LAYOUT>
+static void launch_variant(bool inverse, const BatchArgs &a, dim3 grid, hipStream_t stream) {
+    if (inverse)
+        hipLaunchKernelGGL((fft_batch_kernel<LOG2N, true, NT, LAYOUT>), grid, dim3(batch_wg(LOG2N)), 0, stream, a.in,
+                           a.out, a.tw, a.in_dist, a.out_dist, a.batch);
+    else
+        hipLaunchKernelGGL((fft_batch_kernel<LOG2N, false, NT, LAYOUT>), grid, dim3(batch_wg(LOG2N)), 0, stream, a.in,
+                           a.out, a.tw, a.in_dist, a.out_dist, a.batch);
 }
 
 template <int LOG2N>
-static hipError_t launch_batch_n(bool inverse, const BatchArgs &a, hipStream_t stream) {
+static hipError_t launch_batch_n(bool inverse, const BatchArgs &a, BatchVariant v, hipStream_t stream) {
     constexpr int PER = batch_per_wg(LOG2N);
     const uint64_t blocks = (a.batch + PER - 1) / PER;
     if (blocks == 0) return hipSuccess;
     if (blocks > 0x7fffffffu) return hipErrorInvalidValue;
-    dim3 grid((uint32_t)blocks), block(batch_wg(LOG2N));
-    if (inverse)
-        hipLaunchKernelGGL((fft_batch_kernel<LOG2N, true>), grid, block, 0, stream, a.in, a.out, a.tw, a.in_dist,
-                           a.out_dist, a.batch);
-    else
-        hipLaunchKernelGGL((fft_batch_kernel<LOG2N, false>), grid, block, 0, stream, a.in, a.out, a.tw, a.in_dist,
-                           a.out_dist, a.batch);
+    const dim3 grid((uint32_t)blocks);
+    constexpr bool USES_LDS = stage_count(LOG2N, batch_log2p(LOG2N)) > 1;
+    const bool swz = USES_LDS && v.layout == kSwizzled;
+    if (v.nt) {
+        if (swz) launch_variant<LOG2N, true, kSwizzled>(inverse, a, grid, stream);
+        else launch_variant<LOG2N, true, kPadded>(inverse, a, grid, stream);
+    } else {
+        if (swz) launch_variant<LOG2N, false, kSwizzled>(inverse, a, grid, stream);
+        else launch_variant<LOG2N, false, kPadded>(inverse, a, grid, stream);
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_batch(int log2n, bool inverse, const BatchArgs &a, hipStream_t stream) {
+hipError_t launch_batch(int log2n, bool inverse, const BatchArgs &a, BatchVariant v, hipStream_t stream) {
     switch (log2n) {
-    case 0: return launch_batch_n<0>(inverse, a, stream);
-    case 1: return launch_batch_n<1>(inverse, a, stream);
-    case 2: return launch_batch_n<2>(inverse, a, stream);
-    case 3: return launch_batch_n<3>(inverse, a, stream);
-    case 4: return launch_batch_n<4>(inverse, a, stream);
-    case 5: return launch_batch_n<5>(inverse, a, stream);
-    case 6: return launch_batch_n<6>(inverse, a, stream);
-    case 7: return launch_batch_n<7>(inverse, a, stream);
-    case 8: return launch_batch_n<8>(inverse, a, stream);
-    case 9: return launch_batch_n<9>(inverse, a, stream);
-    case 10: return launch_batch_n<10>(inverse, a, stream);
-    case 11: return launch_batch_n<11>(inverse, a, stream);
-    case 12: return launch_batch_n<12>(inverse, a, stream);
-    case 13: return launch_batch_n<13>(inverse, a, stream);
-    case 14: return launch_batch_n<14>(inverse, a, stream);
+    case 0: return launch_batch_n<0>(inverse, a, v, stream);
+    case 1: return launch_batch_n<1>(inverse, a, v, stream);
+    case 2: return launch_batch_n<2>(inverse, a, v, stream);
+    case 3: return launch_batch_n<3>(inverse, a, v, stream);
+    case 4: return launch_batch_n<4>(inverse, a, v, stream);
+    case 5: return launch_batch_n<5>(inverse, a, v, stream);
+    case 6: return launch_batch_n<6>(inverse, a, v, stream);
+    case 7: return launch_batch_n<7>(inverse, a, v, stream);
+    case 8: return launch_batch_n<8>(inverse, a, v, stream);
+    case 9: return launch_batch_n<9>(inverse, a, v, stream);
+    case 10: return launch_batch_n<10>(inverse, a, v, stream);
+    case 11: return launch_batch_n<11>(inverse, a, v, stream);
+    case 12: return launch_batch_n<12>(inverse, a, v, stream);
+    case 13: return launch_batch_n<13>(inverse, a, v, stream);
+    case 14: return launch_batch_n<14>(inverse, a, v, stream);
     default: return hipErrorInvalidValue;
     }
 }

@@ -33,8 +33,15 @@ struct BatchArgs {
     uint64_t batch;
 };
 
+// Kernel variant of the single-pass transform: nontemporal global access, and the LDS layout
+// (fft_common.hpp: kPadded / kSwizzled). The twiddle table does not depend on the variant.
+struct BatchVariant {
+    bool nt = true;
+    int layout = 0;
+};
+
 // One launch: `batch` transforms of 2^log2n points, 0 <= log2n <= kMaxSinglePassLog2.
-hipError_t launch_batch(int log2n, bool inverse, const BatchArgs &a, hipStream_t stream);
+hipError_t launch_batch(int log2n, bool inverse, const BatchArgs &a, BatchVariant v, hipStream_t stream);
 
 // ---- multi-pass (four-step) transforms, n > 2^kMaxSinglePassLog2 ----------------------------
 

@@ -20,6 +20,7 @@
 #include "kernels.hpp"
 #include "runtime.hpp"
 
+using onefft::BatchVariant;
 using onefft::cd;
 using onefft::cf;
 
@@ -121,6 +122,21 @@ int choose_split(unsigned log2n, int *factors) {
     return p;
 }
 
+// Single-pass kernel variant: nontemporal global access with the padded LDS layout, measured
+// fastest on MI355X for 2^8..2^14 points (the swizzled layout's extra occupancy does not pay).
+// ONEFFT_BATCH_VARIANT="nt,layout" (0/1 each) overrides it, for measurement.
+onefft::BatchVariant choose_variant(unsigned) {
+    onefft::BatchVariant v;
+    if (const char *env = std::getenv("ONEFFT_BATCH_VARIANT")) {
+        int nt = 1, layout = 0;
+        if (std::sscanf(env, "%d,%d", &nt, &layout) == 2) {
+            v.nt = nt != 0;
+            v.layout = layout ? 1 : 0;
+        }
+    }
+    return v;
+}
+
 }  // namespace
 
 // ---- plans ----------------------------------------------------------------------------------
@@ -136,6 +152,7 @@ struct onefft_plan {
     int device = 0;
     void *dmem = nullptr;  // the one device allocation holding every table
     const cf *tw_single = nullptr;
+    BatchVariant variant;  // single-pass kernel variant
     int passes = 0;  // 0: single LDS pass
     PassPlan pass[onefft::kMaxPasses];  // in execution order
 };
@@ -186,7 +203,7 @@ int execute_device(const onefft_plan *plan, const cf *d_in, cf *d_out, uint64_t 
     if (batch == 0) return 0;
     if (plan->passes == 0) {
         BatchArgs a{d_in, d_out, plan->tw_single, in_dist, out_dist, batch};
-        hipError_t e = launch_batch((int)plan->log2n, inverse, a, stream);
+        hipError_t e = launch_batch((int)plan->log2n, inverse, a, plan->variant, stream);
         if (e != hipSuccess) {
             set_error("single-pass launch", e);
             return -1;
@@ -224,6 +241,7 @@ extern "C" onefft_plan *onefft_plan_create(size_t n) {
     if (!plan) return nullptr;
     plan->n = n;
     plan->log2n = log2n;
+    plan->variant = choose_variant(log2n);
     if (hipGetDevice(&plan->device) != hipSuccess) {
         set_error("no HIP device");
         delete plan;

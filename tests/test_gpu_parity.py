@@ -95,7 +95,6 @@ def test_plan_host_pointers(L):
     for inverse in (False, True):
         dst.data[:] = 0
         (p.inverse_ptr if inverse else p.forward_ptr)(src.ptr, dst.ptr)
-        assert F.last_error() == "" or True
         check(dst.data, x, n, inverse, f"n=2^{L} {'inv' if inverse else 'fwd'}")
         assert dst.intact() and src.intact() and np.array_equal(src.data, x)
     p.forward_ptr(src.ptr, dst.ptr)
@@ -222,9 +221,10 @@ def test_large_2_27():
     bins = [0, 1, k0, n // 2, n - 1, 12345677]
     want = O.dft_bins_f64(x, n, bins)
     got = np.concatenate([y[2 * b: 2 * b + 2] for b in bins]).astype(np.float64)
-    # a bin carries absolute error ~ eps sqrt(n) log n from the noise plus a relative error on
+    gc, wc = got[0::2] + 1j * got[1::2], want[0::2] + 1j * want[1::2]
+    # a bin carries absolute error ~ eps sqrt(n) log n from the noise, plus a relative error on
     # its own magnitude (the tone bin is n)
-    assert (np.abs(got - want) < 1e-6 * np.sqrt(n) * L + tol_fp64(L) * np.abs(want)).all()
+    assert (np.abs(gc - wc) < 1e-6 * np.sqrt(n) * L + tol_fp64(L) * np.abs(wc)).all()
     del ref
     back = gpu_batch(y, n, 1, True)
     assert O.rel_rms(back, x.astype(np.float64) * n) < 2 * tol_fp64(L)

@@ -38,7 +38,170 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="approximate CPU-baseline budget")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the timed transform launches plus one 1 GiB device copy (for counter runs)")
+    ap.add_argument("--no-extra", action="store_true", help="skip the other workloads")
+    ap.add_argument("--variants", type=str, default=None,
+                    help="instead of the benchmark: interleaved A/B of the single-pass kernel variants at these "
+                         "log2 sizes (comma list), 512 MiB in and out each")
     return ap.parse_args()
+
+
+def _events(torch):
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def time_ms(torch, stream, fn, reps):
+    """Average milliseconds per call of fn() over reps back-to-back calls, HIP events on `stream`."""
+    e0, e1 = _events(torch)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def variants_ab(torch, F, sizes):
+    """Interleaved A/B of the single-pass kernel variants (ONEFFT_BATCH_VARIANT = "nt,layout")."""
+    import statistics
+
+    variants = {"padded": "0,0", "padded_nt": "1,0", "swizzled": "0,1", "swizzled_nt": "1,1"}
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    old = os.environ.get("ONEFFT_BATCH_VARIANT")
+    for L in sizes:
+        n, batch = 1 << L, (1 << 26) >> L
+        x = torch.randn(batch, n, dtype=torch.complex64, device="cuda")
+        y = torch.empty_like(x)
+        plans = {}
+        for k, v in variants.items():
+            os.environ["ONEFFT_BATCH_VARIANT"] = v
+            plans[k] = F.Plan(n)
+        first = None
+        for k, p in plans.items():
+            p.batch_ptr(x.data_ptr(), y.data_ptr(), batch, n, n, False, sp)
+            torch.cuda.synchronize()
+            if first is None:
+                first = y.clone()
+            elif not torch.equal(first, y):
+                raise SystemExit(f"variant {k} differs from the first variant at log2 n = {L}")
+        times = {k: [] for k in plans}
+        for _ in range(6):
+            for k, p in plans.items():
+                times[k].append(time_ms(torch, stream, lambda: p.batch_ptr(x.data_ptr(), y.data_ptr(), batch, n, n,
+                                                                           False, sp), 10))
+        row = {"log2n": L, "batch": batch, "identical_outputs": True}
+        for k, v in times.items():
+            row[k] = {"min_us": round(min(v) * 1e3, 1), "median_us": round(statistics.median(v) * 1e3, 1),
+                      "gb_s_at_min": round(16.0 * n * batch / (min(v) * 1e-3) / 1e9)}
+        print(json.dumps(row), flush=True)
+        for p in plans.values():
+            p.close()
+    if old is None:
+        os.environ.pop("ONEFFT_BATCH_VARIANT", None)
+    else:
+        os.environ["ONEFFT_BATCH_VARIANT"] = old
+
+
+def extra_workloads(torch, F, stream):
+    """The other workloads, each timed on the device with inputs resident in HBM."""
+    sp = stream.cuda_stream
+    out = {}
+    # n = 1024 forward + inverse through the drop-in path (host pointers) and the device path
+    import numpy as np
+
+    n = 1024
+    p = F.plan(n)
+    xh = np.random.default_rng(0).uniform(-1, 1, 2 * n).astype(np.float32)
+    yh = np.empty_like(xh)
+    for _ in range(20):
+        p.forward_ptr(xh.ctypes.data, yh.ctypes.data)
+    t0 = time.perf_counter()
+    reps = 2000
+    for _ in range(reps):
+        p.forward_ptr(xh.ctypes.data, yh.ctypes.data)
+        p.inverse_ptr(yh.ctypes.data, xh.ctypes.data)
+    host_us = (time.perf_counter() - t0) / reps * 1e6
+    xd = torch.from_numpy(xh).cuda()
+    yd = torch.empty_like(xd)
+    dev_ms = time_ms(torch, stream, lambda: p.batch_ptr(xd.data_ptr(), yd.data_ptr(), 1, n, n, False, sp), 2000)
+    out["n1024"] = {"host_fwd_inv_us": round(host_us, 2), "device_fwd_us": round(dev_ms * 1e3, 2),
+                    "note": "host: synchronous onefft_plan_forward+inverse on host buffers (PCIe copies included)"}
+
+    # large single transforms, forward then inverse
+    large = []
+    for L in (22, 24, 27):
+        n = 1 << L
+        x = torch.randn(n, dtype=torch.complex64, device="cuda")
+        y = torch.empty_like(x)
+        z = torch.empty_like(x)
+        pl = F.Plan(n)
+        fwd = lambda: pl.batch_ptr(x.data_ptr(), y.data_ptr(), 1, n, n, False, sp)
+        inv = lambda: pl.batch_ptr(y.data_ptr(), z.data_ptr(), 1, n, n, True, sp)
+        fwd(), inv()
+        torch.cuda.synchronize()
+        reps = 20 if L < 27 else 5
+        tf, ti = time_ms(torch, stream, fwd, reps), time_ms(torch, stream, inv, reps)
+        rt = (torch.linalg.vector_norm(z / n - x) / torch.linalg.vector_norm(x)).item()
+        large.append({"log2n": L, "fwd_ms": round(tf, 4), "inv_ms": round(ti, 4),
+                      "fwd_gflops": round(5 * n * L / (tf * 1e-3) / 1e9, 1),
+                      "fwd_alg_gb_s": round(16 * n / (tf * 1e-3) / 1e9, 1), "roundtrip_rel_l2": rt,
+                      "working_set": "MALL-resident (<= 256 MB)" if 16 * n <= (256 << 20) else "HBM"})
+        pl.close()
+        del x, y, z
+    out["large"] = large
+
+    # ragged batch: the seed-3 item stream, longest prefix whose input fits 4 GiB
+    items = F.ragged_items(3, 200000, max_bytes=4 << 30)
+    extent = items[-1][0] + (1 << items[-1][1])
+    x = torch.randn(extent, dtype=torch.complex64, device="cuda")
+    y = torch.empty_like(x)
+    rp = F.RaggedPlan(items)
+    one = lambda: rp.execute(x.data_ptr(), y.data_ptr(), sp)
+    one()
+    rp.wait(sp)
+    t_one = time_ms(torch, stream, one, 5)
+    rp.wait(sp)
+    g = F.Graph.ragged(rp, x.data_ptr(), y.data_ptr())
+    t_graph = time_ms(torch, stream, lambda: g.launch(sp), 5)
+    rp.wait(sp)
+    per = lambda: rp.execute_per_item(x.data_ptr(), y.data_ptr(), sp)
+    per()
+    torch.cuda.synchronize()
+    t_per = time_ms(torch, stream, per, 2)
+    flops = sum(5.0 * (1 << l) * l for _, l, _ in items)
+    out["ragged"] = {"items": len(items), "gib_in": round(extent * 8 / 2**30, 3), "tasks": rp.tasks,
+                     "one_launch_ms": round(t_one, 3), "graph_ms": round(t_graph, 3), "per_item_ms": round(t_per, 3),
+                     "one_launch_gflops": round(flops / (t_one * 1e-3) / 1e9, 1),
+                     "alg_gb_s": round(16.0 * extent / (t_one * 1e-3) / 1e9, 1),
+                     "speedup_vs_per_item": round(t_per / t_one, 2),
+                     "interpretation": "seed-3 items (log2 n ~ U{4..20}, 50/50 directions), longest prefix "
+                                       "with input <= 4 GiB, packed back to back"}
+    g.close()
+    rp.close()
+    del x, y
+
+    # spectral filter step: forward, filter, inverse, and the backward (two more transforms)
+    B, n = 256, 65536
+    x = torch.randn(B, n, dtype=torch.complex64, device="cuda", requires_grad=True)
+    H = (1 + 0.1 * torch.randn(n, dtype=torch.complex64, device="cuda")).requires_grad_(True)
+    t = torch.randn(B, n, dtype=torch.complex64, device="cuda")
+
+    def filt():
+        yv = F.fft_autograd(H * F.fft_autograd(x), inverse=True) / n
+        d = yv - t
+        (d.real ** 2 + d.imag ** 2).mean().backward()
+
+    filt()
+    torch.cuda.synchronize()
+    t_f = time_ms(torch, stream, filt, 5)
+    pf = F.plan(n)
+    u = torch.empty_like(t)
+    t_fft = time_ms(torch, stream, lambda: pf.batch_ptr(t.data_ptr(), u.data_ptr(), B, n, n, False, sp), 10)
+    out["filter"] = {"batch": B, "n": n, "step_ms": round(t_f, 3), "fft_ms": round(t_fft, 4),
+                     "fft_gflops": round(5 * n * 16 * B / (t_fft * 1e-3) / 1e9, 1),
+                     "fft_alg_gb_s": round(16 * n * B / (t_fft * 1e-3) / 1e9, 1),
+                     "note": "step = 4 transforms (2 forward, 2 backward) plus torch's elementwise ops"}
+    return out
 
 
 def traffic_from_profiles(name: str):
@@ -89,6 +252,9 @@ def main():
 
     import onefft_amd as F
 
+    if args.variants:
+        variants_ab(torch, F, [int(v) for v in args.variants.split(",")])
+        return
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     g = torch.Generator(device=dev)
@@ -156,6 +322,7 @@ def main():
     torch.cuda.synchronize()
     copy_gbs = alg_bytes / (c0.elapsed_time(c1) / 5 * 1e-3) / 1e9
 
+    extras = {} if args.no_extra else extra_workloads(torch, F, stream)
     cpu = {"value": None, "skipped": True} if args.no_cpu else cpu_baseline(args.cpu_seconds)
 
     line = {
@@ -183,6 +350,7 @@ def main():
         "copy_gb_s": round(copy_gbs, 1),
         "roundtrip_rel_l2": rt,
         "wall_s_timed": round(wall, 4),
+        **extras,
     }
     print(json.dumps(line), flush=True)
 

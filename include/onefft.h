@@ -135,9 +135,25 @@ int onefft_ragged_execute(const onefft_ragged_plan *plan, const float *d_in, flo
 int onefft_ragged_execute_per_item(const onefft_ragged_plan *plan, const float *d_in, float *d_out,
                                    struct ihipStream_t *stream);
 
-/* Waits for `stream` and reports whether the last onefft_ragged_execute of this plan completed
- * correctly: 0, or -1 with onefft_last_error() set. */
+/* Waits for `stream` and reports whether the work queued on it (such as onefft_ragged_execute)
+ * completed: 0, or -1 with onefft_last_error() set. */
 int onefft_ragged_wait(const onefft_ragged_plan *plan, struct ihipStream_t *stream);
+
+/* ---- HIP graphs: a plan execution recorded once, replayed with one launch ------------------- */
+
+typedef struct onefft_graph onefft_graph;
+
+/* Records the launches of one batched device transform (arguments as onefft_plan_*_batch) or of
+ * one ragged execution, on the current device. The buffers are bound into the graph. NULL on
+ * invalid arguments or capture failure. */
+onefft_graph *onefft_graph_create_batch(const onefft_plan *plan, const float *d_in, float *d_out, size_t batch,
+                                        size_t in_dist, size_t out_dist, int inverse);
+onefft_graph *onefft_graph_create_ragged(const onefft_ragged_plan *plan, const float *d_in, float *d_out);
+
+/* Replays the recorded launches on `stream`, asynchronously. Returns 0 or -1. A ragged graph, like
+ * its plan, runs one replay at a time. */
+int onefft_graph_launch(const onefft_graph *graph, struct ihipStream_t *stream);
+void onefft_graph_destroy(onefft_graph *graph);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -47,6 +47,10 @@ SIGNATURES = {
     "onefft_ragged_execute": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "onefft_ragged_execute_per_item": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "onefft_ragged_wait": (c_int, [c_void_p, c_void_p]),
+    "onefft_graph_create_batch": (c_void_p, [c_void_p, c_void_p, c_void_p, c_size_t, c_size_t, c_size_t, c_int]),
+    "onefft_graph_create_ragged": (c_void_p, [c_void_p, c_void_p, c_void_p]),
+    "onefft_graph_launch": (c_int, [c_void_p, c_void_p]),
+    "onefft_graph_destroy": (None, [c_void_p]),
 }
 
 
@@ -200,6 +204,37 @@ def ragged_items(seed: int, count: int, lo: int = 4, hi: int = 20, max_bytes: in
         items.append((off, log2n, inverse))
         off += 1 << log2n
     return items
+
+
+class Graph:
+    """An onefft_graph: one recorded plan execution, replayed with launch()."""
+
+    def __init__(self, handle):
+        if not handle:
+            raise OneFFTError(f"graph creation failed: {last_error()}")
+        self._g = handle
+
+    @classmethod
+    def batch(cls, plan: Plan, src: int, dst: int, batch: int, in_dist: int, out_dist: int, inverse: bool):
+        return cls(lib().onefft_graph_create_batch(plan.handle, src, dst, batch, in_dist, out_dist, int(inverse)))
+
+    @classmethod
+    def ragged(cls, plan: "RaggedPlan", src: int, dst: int):
+        return cls(lib().onefft_graph_create_ragged(plan._p, src, dst))
+
+    def launch(self, stream: int = 0) -> None:
+        _check(lib().onefft_graph_launch(self._g, stream), "onefft_graph_launch")
+
+    def close(self) -> None:
+        if self._g:
+            lib().onefft_graph_destroy(self._g)
+            self._g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _plans: dict[int, Plan] = {}

@@ -19,7 +19,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libonefft_amd.so")
-SOURCES = ["runtime.hip", "staging.hip", "ragged.hip", "k_batch.hip", "k_pass.hip", "k_ragged.hip"]
+SOURCES = ["runtime.hip", "staging.hip", "ragged.hip", "graph.hip", "k_batch.hip", "k_pass.hip", "k_ragged.hip"]
 ARCH = "gfx950"
 
 

@@ -3,16 +3,15 @@
 //
 // The reference transforms one array per call (include/onefft.h:61-77); a caller with many
 // arrays of mixed sizes calls it once per array. Here every array of a batch is an item
-// (offset, log2 n, direction), and the host orders the tasks so that no wait can deadlock:
-//
-//   1. first-pass tiles of the two-pass items (2^13..2^20 points), largest items first;
-//   2. single-pass tasks (up to 4096 / n items of one size n <= 2^12 per task);
-//   3. second-pass tiles, in the same item order as 1.
-//
-// A second-pass tile waits until every first-pass tile of its item has finished. Tasks are
-// claimed in queue order by running workgroups, so every first-pass tile it waits for has already
-// been claimed by a running workgroup that waits for nothing: the wait always ends. It is bounded
-// anyway; a wait that gives up sets `status` (the host reports it as an error).
+// (offset, log2 n, direction). A task is either
+//   * up to 4096 / n whole single-pass items of one size n <= 2^12, or
+//   * one whole two-pass item (2^13..2^20 points), both passes run by the one workgroup that
+//     claimed it, tile after tile (C = 4096 / N_f columns per tile).
+// No task depends on another: a workgroup only ever waits at its own barriers, and global data
+// written in a two-pass item's first pass is read back only by the same workgroup, so the
+// workgroup barrier orders it (no device-scope fences, which on this chip write back and
+// invalidate whole L2 caches). The host puts the largest items first, so the long tasks start
+// early and the queue ends with short ones.
 #include "fft_bodies.hpp"
 
 namespace onefft {
@@ -30,7 +29,6 @@ constexpr uint32_t ragged_lds_values() {
     return m;
 }
 constexpr uint32_t kLds = ragged_lds_values();
-constexpr uint32_t kSpinLimit = 1u << 25;
 
 // One column transform of N = 2^L points (L <= 12) with every stage's shape chosen at run time:
 // a first stage of radix R0 (N itself below 16 points, else 2^(L mod 4), or 16), then radix-16
@@ -151,17 +149,13 @@ __device__ void single_task(const RaggedArgs &a, const RaggedTask &tk, uint32_t 
     column_fft(io, log2n, a.tw32 + tk.tw, lds + c * lds_span(N), t % T);
 }
 
-// one tile (C = 4096 / N_f columns) of one pass of a two-pass transform (index scheme of k_pass.hip)
-__device__ void two_pass_task(const RaggedArgs &a, const RaggedTask &tk, uint32_t kind, uint32_t log2n, bool inv,
-                              cf *lds, uint32_t t) {
-    const RaggedSize &z = a.size[log2n];
-    const uint64_t off = a.item_off[tk.item];
-    const uint32_t tiles = 1u << (log2n - 12);  // tiles per pass
-    const bool first = kind == kRaggedPass1;
+// one tile (C = 4096 / N_f columns) of one pass of a two-pass item (index scheme of k_pass.hip)
+__device__ void pass_tile(const RaggedArgs &a, const RaggedSize &z, const cf *in, cf *out, uint32_t tw, bool first,
+                          uint32_t tile, bool inv, cf *lds, uint32_t t) {
     const uint32_t lf = first ? z.l2 : z.l1;     // this pass's sub-transform
     const uint32_t C = kRaggedTaskPoints >> lf;  // columns per tile
     const uint32_t c = t % C, j = t / C;
-    const uint32_t col = tk.arg * C + c;
+    const uint32_t col = tile * C + c;
     Column io;
     io.live = true;
     io.twiddle = !first;
@@ -171,40 +165,46 @@ __device__ void two_pass_task(const RaggedArgs &a, const RaggedTask &tk, uint32_
     io.shift = 0;
     if (first) {
         // factor 2 (N_2 points) first: column j_1 of the input (stride N_1) -> row at N_2 j_1
-        io.src = a.in + off + col;
+        io.src = in + col;
         io.in_step = 1u << z.l1;
-        io.dst = a.out + off + ((uint64_t)col << z.l2);
+        io.dst = out + ((uint64_t)col << z.l2);
         io.out_step = 1;
         io.swap_in = inv;
         io.swap_out = false;
         io.low = 0;
     } else {
-        if (t == 0) {
-            uint32_t spins = 0;
-            while (__hip_atomic_load(&a.counters[tk.item], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < tiles) {
-                __builtin_amdgcn_s_sleep(2);
-                if (++spins >= kSpinLimit) {
-                    __hip_atomic_fetch_or(a.status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
-                }
-            }
-        }
-        __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         // factor 1 (N_1 points) in place: column k_2 at stride N_2, twiddles w_n^(j_1 k_2)
-        io.src = a.out + off + col;
-        io.dst = a.out + off + col;
+        io.src = out + col;
+        io.dst = out + col;
         io.in_step = io.out_step = 1u << z.l2;
         io.swap_in = false;
         io.swap_out = inv;
         io.low = col;
     }
-    column_fft(io, lf, a.tw32 + tk.tw, lds + c * pass_span((int)lf), j);
-    if (first) {
-        // publish: every thread's stores are visible device-wide before the count goes up
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    column_fft(io, lf, a.tw32 + tw, lds + c * pass_span((int)lf), j);
+}
+
+// a whole two-pass item: every first-pass tile, a barrier, every second-pass tile
+__device__ void two_pass_item(const RaggedArgs &a, const RaggedTask &tk, uint32_t log2n, bool inv, cf *lds,
+                              uint32_t t) {
+    const RaggedSize &z = a.size[log2n];
+    const uint64_t off = a.item_off[tk.item];
+    const uint32_t tiles = 1u << (log2n - 12);  // tiles per pass
+    // The table offsets go through an empty asm statement each tile, so the compiler treats them
+    // as new values and reloads the twiddles per tile instead of holding every tile's twiddles in
+    // registers across the loop (which would cost occupancy).
+    for (uint32_t tile = 0; tile < tiles; tile++) {
+        uint32_t tw = tk.tw;
+        asm volatile("" : "+v"(tw));
+        pass_tile(a, z, a.in + off, a.out + off, tw, true, tile, inv, lds, t);
+        __syncthreads();  // LDS reuse
+    }
+    // the first pass's global stores, made by this workgroup, are complete and visible to it
+    for (uint32_t tile = 0; tile < tiles; tile++) {
+        uint32_t tw = tk.arg;
+        asm volatile("" : "+v"(tw));
+        pass_tile(a, z, a.in + off, a.out + off, tw, false, tile, inv, lds, t);
         __syncthreads();
-        if (t == 0) __hip_atomic_fetch_add(&a.counters[tk.item], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -226,20 +226,14 @@ __global__ __launch_bounds__(kRaggedWG) void fft_ragged_kernel(RaggedArgs a) {
         if (kind == kRaggedSingle) {
             single_task(a, tk, log2n, inv, lds, t);
         } else {
-            two_pass_task(a, tk, kind, log2n, inv, lds, t);
+            two_pass_item(a, tk, log2n, inv, lds, t);
         }
         __syncthreads();  // LDS is free for the next task
     }
 }
 
-__global__ __launch_bounds__(256) void fft_ragged_reset(uint32_t *counters, uint32_t count, uint32_t *head,
-                                                        uint32_t *status) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        *head = 0;
-        *status = 0;
-    }
-    for (uint32_t k = i; k < count; k += gridDim.x * blockDim.x) counters[k] = 0;
+__global__ __launch_bounds__(64) void fft_ragged_reset(uint32_t *head) {
+    if (threadIdx.x == 0) *head = 0;
 }
 
 uint32_t ragged_resident_grid() {
@@ -255,9 +249,7 @@ uint32_t ragged_resident_grid() {
 }
 
 hipError_t launch_ragged(const RaggedArgs &a, uint32_t grid, hipStream_t stream) {
-    const uint32_t reset_blocks = a.nitems / 256 + 1 < 1024 ? a.nitems / 256 + 1 : 1024;
-    hipLaunchKernelGGL(fft_ragged_reset, dim3(reset_blocks), dim3(256), 0, stream, a.counters, a.nitems, a.head,
-                       a.status);
+    hipLaunchKernelGGL(fft_ragged_reset, dim3(1), dim3(64), 0, stream, a.head);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (a.ntasks == 0) return hipSuccess;

@@ -80,22 +80,23 @@ hipError_t launch_pass(int log2nf, bool first, bool swap_in, bool swap_out, cons
 
 // ---- ragged batches: one persistent launch over items of mixed sizes -------------------------
 //
-// Every task is 4096 points of work for a 256-thread workgroup: either up to 4096 / n whole
-// transforms of one size n <= 2^12 (single-pass), or one tile of one pass of a two-pass transform
-// (2^13..2^20 points), C = 4096 / N_f columns of an N_f-point sub-transform.
+// A task is either up to 4096 / n whole transforms of one size n <= 2^12 (single-pass), or one
+// whole two-pass transform of 2^13..2^20 points, run by one 256-thread workgroup in tiles of
+// 4096 points (C = 4096 / N_f columns of an N_f-point sub-transform).
 constexpr int kRaggedMaxLog2 = 20;
 constexpr int kRaggedSingleMaxLog2 = 12;
 constexpr int kRaggedWG = 256;
 constexpr uint32_t kRaggedTaskPoints = 4096;
 
-enum RaggedKind : uint32_t { kRaggedSingle = 0, kRaggedPass1 = 1, kRaggedPass2 = 2 };
+enum RaggedKind : uint32_t { kRaggedSingle = 0, kRaggedTwoPass = 1 };
 
 struct RaggedTask {
     uint32_t word;  // kind | log2n << 2 | inverse << 7
-    uint32_t item;  // single: first index into the sorted item list; pass: the item's index
-    uint32_t arg;   // single: number of transforms; pass: tile index
-    uint32_t tw;    // offset of the stage table this task reads, in the FP32 table blob (read per
-                    // task, so the compiler cannot hoist twiddle loads out of the task loop)
+    uint32_t item;  // single: first index into the sorted item list; two-pass: the item's index
+    uint32_t arg;   // single: number of transforms; two-pass: stage table of N_1 (second pass)
+    uint32_t tw;    // stage table this task reads first (single: of n; two-pass: of N_2), as an
+                    // offset in the FP32 table blob, read per task so the compiler cannot hoist
+                    // twiddle loads out of the task loop
 };
 
 struct RaggedSize {
@@ -111,9 +112,7 @@ struct RaggedArgs {
     cf *out;
     const RaggedTask *tasks;
     const uint64_t *item_off;  // complex offset of each item, in task order
-    uint32_t *counters;        // finished first-pass tiles per item
     uint32_t *head;            // next task to claim
-    uint32_t *status;          // nonzero: a wait gave up (a bug; the result is wrong)
     const cf *tw32;
     const cd *tw64;
     uint32_t ntasks;
@@ -121,8 +120,8 @@ struct RaggedArgs {
     RaggedSize size[kRaggedMaxLog2 + 1];
 };
 
-// Zeroes the queue head, status and item counters, then runs the persistent kernel on `grid`
-// workgroups (0: as many as fit the device at once).
+// Zeroes the queue head (with a kernel, so the pair can be captured in a graph), then runs the
+// persistent kernel on `grid` workgroups (0: as many as fit the device at once).
 hipError_t launch_ragged(const RaggedArgs &a, uint32_t grid, hipStream_t stream);
 // Workgroups of the persistent kernel that fit the device at once.
 uint32_t ragged_resident_grid();

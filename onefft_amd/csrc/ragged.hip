@@ -146,8 +146,8 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
     auto word = [](uint32_t kind, uint32_t log2n, uint32_t inv) { return kind | (log2n << 2) | (inv << 7); };
     for (size_t k = 0; k < nbig; k++) {
         const onefft_ragged_item &it = items[order[k]];
-        for (uint32_t tile = 0; tile < (1u << (it.log2n - 12)); tile++)
-            tasks.push_back({word(kRaggedPass1, it.log2n, it.inverse), (uint32_t)k, tile, a.size[it.log2n].tw2});
+        const RaggedSize &z = a.size[it.log2n];
+        tasks.push_back({word(kRaggedTwoPass, it.log2n, it.inverse), (uint32_t)k, z.tw1, z.tw2});
     }
     for (size_t k = nbig; k < count;) {
         const onefft_ragged_item &it = items[order[k]];
@@ -158,24 +158,18 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
         tasks.push_back({word(kRaggedSingle, it.log2n, it.inverse), (uint32_t)k, (uint32_t)(e - k), a.size[it.log2n].tw});
         k = e;
     }
-    for (size_t k = 0; k < nbig; k++) {
-        const onefft_ragged_item &it = items[order[k]];
-        for (uint32_t tile = 0; tile < (1u << (it.log2n - 12)); tile++)
-            tasks.push_back({word(kRaggedPass2, it.log2n, it.inverse), (uint32_t)k, tile, a.size[it.log2n].tw1});
-    }
     if (tasks.size() > 0xffffffffu) {
         set_error("too many tasks");
         delete plan;
         return nullptr;
     }
 
-    // one device allocation: tasks | item offsets | counters | head, status | tw32 | tw64
+    // one device allocation: tasks | item offsets | queue head | tw32 | tw64
     auto up = [](size_t v) { return (v + 255) / 256 * 256; };
     const size_t o_tasks = 0;
     const size_t o_off = up(o_tasks + tasks.size() * sizeof(RaggedTask));
-    const size_t o_cnt = up(o_off + item_off.size() * sizeof(uint64_t));
-    const size_t o_head = up(o_cnt + (nbig ? nbig : 1) * sizeof(uint32_t));
-    const size_t o_tw32 = up(o_head + 2 * sizeof(uint32_t));
+    const size_t o_head = up(o_off + item_off.size() * sizeof(uint64_t));
+    const size_t o_tw32 = up(o_head + sizeof(uint32_t));
     const size_t o_tw64 = up(o_tw32 + tw32.size() * sizeof(cf));
     const size_t bytes = up(o_tw64 + tw64.size() * sizeof(cd));
     std::vector<unsigned char> image(bytes, 0);
@@ -199,13 +193,11 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
     unsigned char *d = static_cast<unsigned char *>(plan->dmem);
     a.tasks = reinterpret_cast<const RaggedTask *>(d + o_tasks);
     a.item_off = reinterpret_cast<const uint64_t *>(d + o_off);
-    a.counters = reinterpret_cast<uint32_t *>(d + o_cnt);
     a.head = reinterpret_cast<uint32_t *>(d + o_head);
-    a.status = a.head + 1;
     a.tw32 = reinterpret_cast<const cf *>(d + o_tw32);
     a.tw64 = reinterpret_cast<const cd *>(d + o_tw64);
     a.ntasks = (uint32_t)tasks.size();
-    a.nitems = (uint32_t)nbig;
+    a.nitems = (uint32_t)count;
     plan->grid = ragged_resident_grid();
     // the per-item comparison uses the cached plan of every size
     for (int l = 0; l <= kRaggedMaxLog2; l++)
@@ -275,16 +267,6 @@ extern "C" int onefft_ragged_wait(const onefft_ragged_plan *plan, struct ihipStr
     hipError_t e = hipStreamSynchronize(stream);
     if (e != hipSuccess) {
         set_error("ragged execution", e);
-        return -1;
-    }
-    uint32_t status = 0;
-    e = hipMemcpy(&status, plan->args.status, sizeof(status), hipMemcpyDeviceToHost);
-    if (e != hipSuccess) {
-        set_error("ragged status read", e);
-        return -1;
-    }
-    if (status) {
-        set_error("ragged execution: a second-pass wait gave up");
         return -1;
     }
     return 0;

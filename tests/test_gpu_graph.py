@@ -1,0 +1,68 @@
+"""Plan executions captured as HIP graphs: every replay equals the eager result bitwise, with the
+program's own eager work and synchronisations between replays."""
+import numpy as np
+import pytest
+import torch
+
+import onefft_amd as F
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+@pytest.mark.parametrize("L,batch,inverse", [(10, 300, False), (12, 64, True), (16, 3, False), (20, 1, True)])
+def test_batch_graph_replays(L, batch, inverse):
+    n = 1 << L
+    x = torch.from_numpy(O.normal(50 + L, n * batch)).cuda()
+    eager, out = torch.empty_like(x), torch.empty_like(x)
+    p = F.plan(n)
+    p.batch_ptr(x.data_ptr(), eager.data_ptr(), batch, n, n, inverse, stream())
+    torch.cuda.synchronize()
+    g = F.Graph.batch(p, x.data_ptr(), out.data_ptr(), batch, n, n, inverse)
+    other = torch.empty_like(x)
+    for replay in range(4):
+        out.fill_(7.0)
+        torch.cuda.synchronize()
+        g.launch(stream())
+        torch.cuda.synchronize()
+        assert torch.equal(out, eager), replay
+        # eager work between replays: a different transform into another buffer
+        p.batch_ptr(x.data_ptr(), other.data_ptr(), batch, n, n, not inverse, stream())
+        torch.cuda.synchronize()
+    g.close()
+
+
+def test_ragged_graph_replays():
+    items = F.ragged_items(9, 400, lo=2, hi=17)
+    extent = items[-1][0] + (1 << items[-1][1])
+    x = torch.from_numpy(O.normal(9, extent)).cuda()
+    eager, out = torch.zeros_like(x), torch.zeros_like(x)
+    rp = F.RaggedPlan(items)
+    rp.execute(x.data_ptr(), eager.data_ptr(), stream())
+    rp.wait(stream())
+    g = F.Graph.ragged(rp, x.data_ptr(), out.data_ptr())
+    for replay in range(4):
+        out.zero_()
+        torch.cuda.synchronize()
+        g.launch(stream())
+        rp.wait(stream())
+        assert torch.equal(out, eager), replay
+        rp.execute(x.data_ptr(), eager.data_ptr(), stream())  # eager work between replays
+        rp.wait(stream())
+    # the eager ragged result is itself right
+    y, xs = eager.cpu().numpy(), x.cpu().numpy()
+    for off, l, inv in items[:50]:
+        seg = slice(2 * off, 2 * (off + (1 << l)))
+        assert O.rel_rms(y[seg], O.fft_f64(xs[seg], 1 << l, bool(inv))) < 4e-7 + 1e-7 * l
+
+
+def test_graph_rejects_bad_arguments():
+    lib = F.lib()
+    assert not lib.onefft_graph_create_batch(None, None, None, 1, 1, 1, 0)
+    assert not lib.onefft_graph_create_ragged(None, None, None)
+    assert lib.onefft_graph_launch(None, None) == -1
+    lib.onefft_graph_destroy(None)

@@ -19,7 +19,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libonefft_amd.so")
-SOURCES = ["runtime.hip", "staging.hip", "ragged.hip", "graph.hip", "k_batch.hip", "k_pass.hip", "k_ragged.hip"]
+SOURCES = ["runtime.hip", "staging.hip", "ragged.hip", "graph.hip", "tuning.hip", "k_batch.hip", "k_pass.hip", "k_ragged.hip"]
 ARCH = "gfx950"
 
 
@@ -59,7 +59,7 @@ def build(verbose: bool = True) -> str:
         f.write("{ global: onefft_*; local: *; };\n")
     tmp = LIB + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, f"-Wl,--version-script={script}",
-           "-o", tmp]
+           "-ldl", "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

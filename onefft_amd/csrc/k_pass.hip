@@ -21,9 +21,8 @@
 
 namespace onefft {
 
-template <int LOG2NF, bool FIRST, bool SWAP_IN, bool SWAP_OUT>
-__global__ __launch_bounds__(pass_wg(LOG2NF)) void fft_pass_kernel(PassArgs a) {
-    constexpr int C = pass_cols(LOG2NF);
+template <int LOG2NF, int C, bool FIRST, bool SWAP_IN, bool SWAP_OUT>
+__global__ __launch_bounds__(pass_wg(LOG2NF, C)) void fft_pass_kernel(PassArgs a) {
     constexpr bool USES_LDS = stage_count(LOG2NF, pass_log2p(LOG2NF)) > 1;
     __shared__ cf lds[USES_LDS ? C * pass_span(LOG2NF) : 1];
     pass_body<LOG2NF, C, FIRST, SWAP_IN, SWAP_OUT>(a, a.in + (uint64_t)blockIdx.y * a.in_dist,
@@ -31,39 +30,54 @@ __global__ __launch_bounds__(pass_wg(LOG2NF)) void fft_pass_kernel(PassArgs a) {
                                                    threadIdx.x);
 }
 
-template <int LOG2NF>
-static hipError_t launch_pass_n(bool first, bool swap_in, bool swap_out, const PassArgs &a, uint64_t batch,
-                                hipStream_t stream) {
-    constexpr uint32_t C = pass_cols(LOG2NF);
-    if (a.columns % C || batch == 0 || batch > 65535) return hipErrorInvalidValue;
-    dim3 grid(a.columns / C, (uint32_t)batch), block(pass_wg(LOG2NF));
-    if (first) {
-        if (swap_in)
-            hipLaunchKernelGGL((fft_pass_kernel<LOG2NF, true, true, false>), grid, block, 0, stream, a);
-        else
-            hipLaunchKernelGGL((fft_pass_kernel<LOG2NF, true, false, false>), grid, block, 0, stream, a);
+template <int LOG2NF, int C>
+static hipError_t launch_pass_nc(bool first, bool swap_in, bool swap_out, const PassArgs &a, uint64_t batch,
+                                 hipStream_t stream) {
+    if constexpr (!pass_cols_ok(LOG2NF, C)) {
+        return hipErrorInvalidValue;
     } else {
-        if (swap_out)
-            hipLaunchKernelGGL((fft_pass_kernel<LOG2NF, false, false, true>), grid, block, 0, stream, a);
-        else
-            hipLaunchKernelGGL((fft_pass_kernel<LOG2NF, false, false, false>), grid, block, 0, stream, a);
+        if (a.columns % C || batch == 0 || batch > 65535) return hipErrorInvalidValue;
+        dim3 grid(a.columns / C, (uint32_t)batch), block(pass_wg(LOG2NF, C));
+        if (first) {
+            if (swap_in)
+                hipLaunchKernelGGL((fft_pass_kernel<LOG2NF, C, true, true, false>), grid, block, 0, stream, a);
+            else
+                hipLaunchKernelGGL((fft_pass_kernel<LOG2NF, C, true, false, false>), grid, block, 0, stream, a);
+        } else {
+            if (swap_out)
+                hipLaunchKernelGGL((fft_pass_kernel<LOG2NF, C, false, false, true>), grid, block, 0, stream, a);
+            else
+                hipLaunchKernelGGL((fft_pass_kernel<LOG2NF, C, false, false, false>), grid, block, 0, stream, a);
+        }
+        return hipGetLastError();
     }
-    return hipGetLastError();
 }
 
-hipError_t launch_pass(int log2nf, bool first, bool swap_in, bool swap_out, const PassArgs &a, uint64_t batch,
-                       hipStream_t stream) {
+template <int LOG2NF>
+static hipError_t launch_pass_n(int cols, bool first, bool swap_in, bool swap_out, const PassArgs &a, uint64_t batch,
+                                hipStream_t stream) {
+    switch (cols) {
+    case 8: return launch_pass_nc<LOG2NF, 8>(first, swap_in, swap_out, a, batch, stream);
+    case 16: return launch_pass_nc<LOG2NF, 16>(first, swap_in, swap_out, a, batch, stream);
+    case 32: return launch_pass_nc<LOG2NF, 32>(first, swap_in, swap_out, a, batch, stream);
+    case 64: return launch_pass_nc<LOG2NF, 64>(first, swap_in, swap_out, a, batch, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_pass(int log2nf, int cols, bool first, bool swap_in, bool swap_out, const PassArgs &a,
+                       uint64_t batch, hipStream_t stream) {
     // a first pass never stores the final result (there are at least two passes)
     if (first && swap_out) return hipErrorInvalidValue;
     if (!first && swap_in) return hipErrorInvalidValue;
     switch (log2nf) {
-    case 4: return launch_pass_n<4>(first, swap_in, swap_out, a, batch, stream);
-    case 5: return launch_pass_n<5>(first, swap_in, swap_out, a, batch, stream);
-    case 6: return launch_pass_n<6>(first, swap_in, swap_out, a, batch, stream);
-    case 7: return launch_pass_n<7>(first, swap_in, swap_out, a, batch, stream);
-    case 8: return launch_pass_n<8>(first, swap_in, swap_out, a, batch, stream);
-    case 9: return launch_pass_n<9>(first, swap_in, swap_out, a, batch, stream);
-    case 10: return launch_pass_n<10>(first, swap_in, swap_out, a, batch, stream);
+    case 4: return launch_pass_n<4>(cols, first, swap_in, swap_out, a, batch, stream);
+    case 5: return launch_pass_n<5>(cols, first, swap_in, swap_out, a, batch, stream);
+    case 6: return launch_pass_n<6>(cols, first, swap_in, swap_out, a, batch, stream);
+    case 7: return launch_pass_n<7>(cols, first, swap_in, swap_out, a, batch, stream);
+    case 8: return launch_pass_n<8>(cols, first, swap_in, swap_out, a, batch, stream);
+    case 9: return launch_pass_n<9>(cols, first, swap_in, swap_out, a, batch, stream);
+    case 10: return launch_pass_n<10>(cols, first, swap_in, swap_out, a, batch, stream);
     default: return hipErrorInvalidValue;
     }
 }

@@ -49,10 +49,19 @@ constexpr int kMaxPasses = 4;
 constexpr int kPassMinLog2 = 4;   // smallest sub-transform of a pass
 constexpr int kPassMaxLog2 = 10;  // largest sub-transform of a pass
 
-// Points per thread and columns per workgroup of the pass kernel for a 2^log2nf sub-transform.
+// Points per thread of the pass kernel for a 2^log2nf sub-transform, and the columns per
+// workgroup: C adjacent columns make every global access a run of C complex values (C * 8
+// bytes). The default C and the choices a plan may make (LDS of C columns <= 72 KiB).
 constexpr int pass_log2p(int log2nf) { return log2nf < 4 ? log2nf : 4; }
-constexpr int pass_cols(int log2nf) { return log2nf >= 10 ? 8 : 16; }
-constexpr int pass_wg(int log2nf) { return pass_cols(log2nf) << (log2nf - pass_log2p(log2nf)); }
+constexpr bool pass_cols_ok(int log2nf, int cols) {
+    return (cols == 8 || cols == 16 || cols == 32 || cols == 64) && (cols << log2nf) <= 8192 &&
+           (cols << (log2nf - pass_log2p(log2nf))) <= 1024 && (cols << (log2nf - pass_log2p(log2nf))) >= 64;
+}
+// default: 16 columns (128-byte runs) where that is allowed, else the nearest allowed count
+constexpr int pass_cols(int log2nf) {
+    return pass_cols_ok(log2nf, 16) ? 16 : pass_cols_ok(log2nf, 32) ? 32 : pass_cols_ok(log2nf, 64) ? 64 : 8;
+}
+constexpr int pass_wg(int log2nf, int cols) { return cols << (log2nf - pass_log2p(log2nf)); }
 
 struct PassArgs {
     const cf *in;
@@ -74,9 +83,10 @@ struct PassArgs {
 };
 
 // One pass of a multi-pass transform: first = reads the input (out of place), else in place on
-// `out`. swap_in / swap_out exchange re and im on load / store (the inverse).
-hipError_t launch_pass(int log2nf, bool first, bool swap_in, bool swap_out, const PassArgs &a, uint64_t batch,
-                       hipStream_t stream);
+// `out`. swap_in / swap_out exchange re and im on load / store (the inverse). cols: columns per
+// workgroup (pass_cols_ok).
+hipError_t launch_pass(int log2nf, int cols, bool first, bool swap_in, bool swap_out, const PassArgs &a,
+                       uint64_t batch, hipStream_t stream);
 
 // ---- ragged batches: one persistent launch over items of mixed sizes -------------------------
 //

@@ -92,12 +92,41 @@ using onefft::fill_stage_table;
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
-// Pass split of a multi-pass size: log2 N_1 .. log2 N_p (N_1 is transformed last). Even split
-// into passes of at most 2^9 points, the larger factors first in execution order. The
-// environment variable ONEFFT_PASS_SPLIT ("a,b[,c[,d]]", log2 N_1 first) overrides it for sizes
-// whose factors it matches, to sweep splits without rebuilding (the reference's analogue is its
-// ONEFFT_X86_TUNING build switch, src/onefft.c:652-697).
-int choose_split(unsigned log2n, int *factors) {
+// Pass split of a multi-pass size: log2 N_1 .. log2 N_p (N_1 is transformed last), and the
+// columns per workgroup of each factor's pass. The environment variable ONEFFT_PASS_SPLIT
+// ("a,b[,c[,d]]", log2 N_1 first) overrides for sizes whose factors it matches, to sweep splits
+// without rebuilding (the reference's analogue is its ONEFFT_X86_TUNING build switch,
+// src/onefft.c:652-697); else the tuning table's entry; else an even split into passes of at
+// most 2^9 points, the larger factors first in execution order.
+int choose_cols(int log2nf);
+int choose_split_factors(unsigned log2n, int regime, int *factors);
+
+int choose_split(unsigned log2n, int regime, int *factors, int *cols) {
+    const int p = choose_split_factors(log2n, regime, factors);
+    const onefft::TuneEntry &t = onefft::tune_lookup(log2n, regime);
+    bool from_table = t.kind == onefft::TuneEntry::kMulti && t.passes == p;
+    for (int i = 0; i < p && from_table; i++) from_table = t.split[i] == factors[i];
+    for (int i = 0; i < p; i++) cols[i] = from_table ? t.cols[i] : choose_cols(factors[i]);
+    // ONEFFT_PASS_COLS: one count for every pass, or a list "c_1,..,c_p" (N_1 first); counts a
+    // pass does not allow are ignored
+    if (const char *env = std::getenv("ONEFFT_PASS_COLS")) {
+        int c[onefft::kMaxPasses] = {}, k = 0;
+        for (const char *q = env; *q && k < onefft::kMaxPasses;) {
+            char *end;
+            const long v = std::strtol(q, &end, 10);
+            if (end == q) break;
+            c[k++] = (int)v;
+            q = *end == ',' ? end + 1 : end;
+        }
+        for (int i = 0; i < p; i++) {
+            const int want = k == p ? c[i] : c[0];
+            if (k && onefft::pass_cols_ok(factors[i], want)) cols[i] = want;
+        }
+    }
+    return p;
+}
+
+int choose_split_factors(unsigned log2n, int regime, int *factors) {
     if (const char *env = std::getenv("ONEFFT_PASS_SPLIT")) {
         int f[onefft::kMaxPasses], p = 0, sum = 0;
         const char *s = env;
@@ -116,17 +145,25 @@ int choose_split(unsigned log2n, int *factors) {
             return p;
         }
     }
+    const onefft::TuneEntry &t = onefft::tune_lookup(log2n, regime);
+    if (t.kind == onefft::TuneEntry::kMulti) {
+        for (int i = 0; i < t.passes; i++) factors[i] = t.split[i];
+        return t.passes;
+    }
     int p = (int)((log2n + 8) / 9);
     if (p < 2) p = 2;
     for (int i = 0; i < p; i++) factors[i] = (int)log2n / p + (i >= p - (int)log2n % p ? 1 : 0);
     return p;
 }
 
-// Single-pass kernel variant: nontemporal global access with the padded LDS layout, measured
-// fastest on MI355X for 2^8..2^14 points (the swizzled layout's extra occupancy does not pay).
-// ONEFFT_BATCH_VARIANT="nt,layout" (0/1 each) overrides it, for measurement.
-onefft::BatchVariant choose_variant(unsigned) {
+// Single-pass kernel variant: the tuning table's choice, else nontemporal global access with the
+// padded LDS layout (measured fastest on MI355X for 2^8..2^14 points: the swizzled layout's extra
+// occupancy does not pay). ONEFFT_BATCH_VARIANT="nt,layout" (0/1 each) overrides both, for
+// measurement.
+onefft::BatchVariant choose_variant(unsigned log2n, int regime) {
     onefft::BatchVariant v;
+    const onefft::TuneEntry &t = onefft::tune_lookup(log2n, regime);
+    if (t.kind == onefft::TuneEntry::kSingle) v = t.variant;
     if (const char *env = std::getenv("ONEFFT_BATCH_VARIANT")) {
         int nt = 1, layout = 0;
         if (std::sscanf(env, "%d,%d", &nt, &layout) == 2) {
@@ -137,24 +174,35 @@ onefft::BatchVariant choose_variant(unsigned) {
     return v;
 }
 
+// Columns per workgroup of a pass over 2^log2nf-point sub-transforms: the kernels' default.
+int choose_cols(int log2nf) { return onefft::pass_cols(log2nf); }
+
 }  // namespace
 
 // ---- plans ----------------------------------------------------------------------------------
 
 struct PassPlan {
     int log2nf;
+    int cols;  // columns per workgroup
     onefft::PassArgs args;  // pointers to data filled in per call
 };
 
+// The passes of a multi-pass transform, in execution order.
+struct PassConfig {
+    int passes = 0;
+    PassPlan pass[onefft::kMaxPasses];
+};
+
+// Choices are made per working-set regime (runtime.hpp: tune_regime), index 0 or 1.
 struct onefft_plan {
     size_t n = 0;
     unsigned log2n = 0;
     int device = 0;
     void *dmem = nullptr;  // the one device allocation holding every table
+    bool multi = false;    // false: one LDS pass
     const cf *tw_single = nullptr;
-    BatchVariant variant;  // single-pass kernel variant
-    int passes = 0;  // 0: single LDS pass
-    PassPlan pass[onefft::kMaxPasses];  // in execution order
+    BatchVariant variant[2];  // single-pass kernel variant per regime
+    PassConfig cfg[2];        // multi-pass split per regime
 };
 
 namespace onefft {
@@ -201,25 +249,28 @@ struct DeviceGuard {
 int execute_device(const onefft_plan *plan, const cf *d_in, cf *d_out, uint64_t batch, uint64_t in_dist,
                    uint64_t out_dist, bool inverse, hipStream_t stream) {
     if (batch == 0) return 0;
-    if (plan->passes == 0) {
+    const int regime = tune_regime(plan->n, batch);
+    if (!plan->multi) {
         BatchArgs a{d_in, d_out, plan->tw_single, in_dist, out_dist, batch};
-        hipError_t e = launch_batch((int)plan->log2n, inverse, a, plan->variant, stream);
+        hipError_t e = launch_batch((int)plan->log2n, inverse, a, plan->variant[regime], stream);
         if (e != hipSuccess) {
             set_error("single-pass launch", e);
             return -1;
         }
         return 0;
     }
+    const PassConfig &cfg = plan->cfg[regime];
     for (uint64_t done = 0; done < batch;) {
         const uint64_t chunk = batch - done < 65535 ? batch - done : 65535;
-        for (int i = 0; i < plan->passes; i++) {
-            PassArgs a = plan->pass[i].args;
-            const bool first = i == 0, last = i == plan->passes - 1;
+        for (int i = 0; i < cfg.passes; i++) {
+            PassArgs a = cfg.pass[i].args;
+            const bool first = i == 0, last = i == cfg.passes - 1;
             a.in = first ? d_in + done * in_dist : d_out + done * out_dist;
             a.out = d_out + done * out_dist;
             a.in_dist = first ? in_dist : out_dist;
             a.out_dist = out_dist;
-            hipError_t e = launch_pass(plan->pass[i].log2nf, first, first && inverse, last && inverse, a, chunk, stream);
+            hipError_t e = launch_pass(cfg.pass[i].log2nf, cfg.pass[i].cols, first, first && inverse,
+                                       last && inverse, a, chunk, stream);
             if (e != hipSuccess) {
                 set_error("pass launch", e);
                 return -1;
@@ -241,7 +292,8 @@ extern "C" onefft_plan *onefft_plan_create(size_t n) {
     if (!plan) return nullptr;
     plan->n = n;
     plan->log2n = log2n;
-    plan->variant = choose_variant(log2n);
+    plan->variant[0] = choose_variant(log2n, 0);
+    plan->variant[1] = choose_variant(log2n, 1);
     if (hipGetDevice(&plan->device) != hipSuccess) {
         set_error("no HIP device");
         delete plan;
@@ -255,17 +307,15 @@ extern "C" onefft_plan *onefft_plan_create(size_t n) {
         image.resize(off + bytes);
         return off;
     };
-    size_t off_single = 0, off_stage[kMaxPasses] = {}, off_lo = 0, off_hi = 0;
+    size_t off_single = 0, off_stage[2][kMaxPasses] = {}, off_lo = 0, off_hi = 0;
     if (log2n <= (unsigned)kMaxSinglePassLog2) {
         const int lp = batch_log2p((int)log2n);
         const uint32_t entries = stage_tw_size((int)log2n, lp);
         off_single = reserve(entries * sizeof(cf) + sizeof(cf));
         fill_stage_table((int)log2n, lp, reinterpret_cast<cf *>(image.data() + off_single));
     } else {
-        int factors[kMaxPasses];
-        const int p = choose_split(log2n, factors);
-        plan->passes = p;
-        // two-level FP64 table of w_n
+        plan->multi = true;
+        // two-level FP64 table of w_n, shared by both configurations
         const unsigned lo_bits = (log2n + 1) / 2;
         const size_t lo_count = (size_t)1 << lo_bits, hi_count = (size_t)1 << (log2n - lo_bits);
         off_lo = reserve(lo_count * sizeof(cd));
@@ -276,30 +326,37 @@ extern "C" onefft_plan *onefft_plan_create(size_t n) {
             cd *hi = reinterpret_cast<cd *>(image.data() + off_hi);
             for (size_t i = 0; i < hi_count; i++) root64((uint64_t)i << lo_bits, log2n, &hi[i].r, &hi[i].i);
         }
-        // execution order: factor f = p, p-1, ..., 1 (factors[f-1])
-        for (int i = 0; i < p; i++) {
-            const int f = p - i;  // 1-based factor index
-            const int lf = factors[f - 1];
-            const int lp = pass_log2p(lf);
-            off_stage[i] = reserve(stage_tw_size(lf, lp) * sizeof(cf) + sizeof(cf));
-            fill_stage_table(lf, lp, reinterpret_cast<cf *>(image.data() + off_stage[i]));
-            PassArgs &a = plan->pass[i].args;
-            std::memset(&a, 0, sizeof(a));
-            plan->pass[i].log2nf = lf;
-            unsigned log2m = 0;
-            for (int l = f + 1; l <= p; l++) log2m += (unsigned)factors[l - 1];
-            a.lo_bits = lo_bits;
-            a.log2n = log2n;
-            a.log2m = log2m;
-            a.tw_shift = log2n - log2m - (unsigned)lf;
-            a.columns = (uint32_t)(n >> lf);
-            if (i == 0) {
-                a.ndig = (uint32_t)(p - 1);
-                for (int d = 0; d < p - 1; d++) {  // digit j_{d+1}
-                    a.dig_log2[d] = (uint32_t)factors[d];
-                    uint32_t shift = 0;
-                    for (int l = d + 2; l <= p - 1; l++) shift += (uint32_t)factors[l - 1];
-                    a.dig_shift[d] = shift;
+        for (int r = 0; r < 2; r++) {
+            int factors[kMaxPasses], cols[kMaxPasses];
+            const int p = choose_split(log2n, r, factors, cols);
+            PassConfig &cfg = plan->cfg[r];
+            cfg.passes = p;
+            // execution order: factor f = p, p-1, ..., 1 (factors[f-1])
+            for (int i = 0; i < p; i++) {
+                const int f = p - i;  // 1-based factor index
+                const int lf = factors[f - 1];
+                const int lp = pass_log2p(lf);
+                off_stage[r][i] = reserve(stage_tw_size(lf, lp) * sizeof(cf) + sizeof(cf));
+                fill_stage_table(lf, lp, reinterpret_cast<cf *>(image.data() + off_stage[r][i]));
+                PassArgs &a = cfg.pass[i].args;
+                std::memset(&a, 0, sizeof(a));
+                cfg.pass[i].log2nf = lf;
+                cfg.pass[i].cols = cols[f - 1];
+                unsigned log2m = 0;
+                for (int l = f + 1; l <= p; l++) log2m += (unsigned)factors[l - 1];
+                a.lo_bits = lo_bits;
+                a.log2n = log2n;
+                a.log2m = log2m;
+                a.tw_shift = log2n - log2m - (unsigned)lf;
+                a.columns = (uint32_t)(n >> lf);
+                if (i == 0) {
+                    a.ndig = (uint32_t)(p - 1);
+                    for (int d = 0; d < p - 1; d++) {  // digit j_{d+1}
+                        a.dig_log2[d] = (uint32_t)factors[d];
+                        uint32_t shift = 0;
+                        for (int l = d + 2; l <= p - 1; l++) shift += (uint32_t)factors[l - 1];
+                        a.dig_shift[d] = shift;
+                    }
                 }
             }
         }
@@ -319,14 +376,16 @@ extern "C" onefft_plan *onefft_plan_create(size_t n) {
         return nullptr;
     }
     auto dev = [&](size_t off) { return reinterpret_cast<unsigned char *>(plan->dmem) + off; };
-    if (plan->passes == 0) {
+    if (!plan->multi) {
         plan->tw_single = reinterpret_cast<const cf *>(dev(off_single));
     } else {
-        for (int i = 0; i < plan->passes; i++) {
-            PassArgs &a = plan->pass[i].args;
-            a.tw = reinterpret_cast<const cf *>(dev(off_stage[i]));
-            a.tw_lo = reinterpret_cast<const cd *>(dev(off_lo));
-            a.tw_hi = reinterpret_cast<const cd *>(dev(off_hi));
+        for (int r = 0; r < 2; r++) {
+            for (int i = 0; i < plan->cfg[r].passes; i++) {
+                PassArgs &a = plan->cfg[r].pass[i].args;
+                a.tw = reinterpret_cast<const cf *>(dev(off_stage[r][i]));
+                a.tw_lo = reinterpret_cast<const cd *>(dev(off_lo));
+                a.tw_hi = reinterpret_cast<const cd *>(dev(off_hi));
+            }
         }
     }
     staging_reserve(plan->device, n);

@@ -39,6 +39,20 @@ void fill_stage_table(int log2n, int log2p, cf *table);
 // The cache's plan for size n (created on first use), or nullptr.
 const onefft_plan *cached_plan(size_t n);
 
+// One size's entry of the tuning table (tuning.hip), per working-set regime: 0 when the batch's
+// input plus output fit the 256 MiB Infinity Cache (MALL), 1 when they do not.
+constexpr int kTuneMaxLog2 = 30;
+constexpr uint64_t kMallBytes = uint64_t(256) << 20;
+inline int tune_regime(uint64_t n, uint64_t batch) { return 16 * n * batch > kMallBytes ? 1 : 0; }
+struct TuneEntry {
+    enum Kind { kNone, kSingle, kMulti } kind = kNone;
+    BatchVariant variant;        // kSingle
+    int passes = 0;              // kMulti: the split, log2 N_1 first, and columns per pass factor
+    int split[kMaxPasses] = {};
+    int cols[kMaxPasses] = {};
+};
+const TuneEntry &tune_lookup(unsigned log2n, int regime);
+
 // The calling thread's stream on `device` (created on first use), or nullptr on failure.
 hipStream_t thread_stream(int device);
 

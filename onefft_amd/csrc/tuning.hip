@@ -1,0 +1,108 @@
+// The persisted tuning table: per transform size, the single-pass kernel variant or the
+// multi-pass split and columns per workgroup that measured fastest on this GPU.
+//
+// File: tab-separated lines "log2n  kind  params  gb_s  batch", '#' comments, where kind is
+// "single" (params "nt=0|1,layout=0|1") or "multi" (params "split=a,b[,c[,d]];cols=x,y[,..]",
+// log2 N_1 first). The batch the line was measured at sets its working-set regime
+// (runtime.hpp: tune_regime); a size may have a line per regime. Written by `python -m onefft_amd.tune`; read once, at the first plan creation,
+// from $ONEFFT_TUNE_FILE or tune_gfx950.tsv next to the library. Plan creation stays O(n) with no
+// measuring, as with the reference's planner (its tuning is compiled in, src/onefft.c:652-697).
+#include <dlfcn.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "runtime.hpp"
+
+namespace onefft {
+namespace {
+
+TuneEntry g_table[kTuneMaxLog2 + 1][2];
+std::once_flag g_once;
+
+void load_table();
+
+std::string table_path() {
+    if (const char *env = std::getenv("ONEFFT_TUNE_FILE")) return env;
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void *>(&load_table), &info) && info.dli_fname) {
+        std::string p = info.dli_fname;
+        const size_t slash = p.rfind('/');
+        return (slash == std::string::npos ? std::string(".") : p.substr(0, slash)) + "/tune_gfx950.tsv";
+    }
+    return "tune_gfx950.tsv";
+}
+
+// "a,b,c" -> up to `max` ints; returns the count, 0 on a malformed list
+int parse_list(const char *s, int *out, int max) {
+    int count = 0;
+    while (*s && count < max) {
+        char *end;
+        const long v = std::strtol(s, &end, 10);
+        if (end == s) return 0;
+        out[count++] = (int)v;
+        if (*end != ',') break;
+        s = end + 1;
+    }
+    return count;
+}
+
+void parse_line(const char *line) {
+    int log2n = -1;
+    double gbs = 0;
+    unsigned long long batch = 0;
+    char kind[16] = {0}, params[128] = {0};
+    if (std::sscanf(line, "%d %15s %127s %lf %llu", &log2n, kind, params, &gbs, &batch) != 5) return;
+    if (log2n < 0 || log2n > kTuneMaxLog2 || batch == 0) return;
+    const int regime = tune_regime(uint64_t(1) << log2n, batch);
+    TuneEntry e;
+    if (!std::strcmp(kind, "single")) {
+        int nt = 1, layout = 0;
+        if (std::sscanf(params, "nt=%d,layout=%d", &nt, &layout) != 2) return;
+        e.kind = TuneEntry::kSingle;
+        e.variant.nt = nt != 0;
+        e.variant.layout = layout ? 1 : 0;
+    } else if (!std::strcmp(kind, "multi")) {
+        const char *sp = std::strstr(params, "split="), *cp = std::strstr(params, "cols=");
+        if (!sp || !cp) return;
+        e.passes = parse_list(sp + 6, e.split, kMaxPasses);
+        const int nc = parse_list(cp + 5, e.cols, kMaxPasses);
+        if (e.passes < 2 || nc != e.passes) return;
+        int sum = 0;
+        for (int i = 0; i < e.passes; i++) {
+            if (e.split[i] < kPassMinLog2 || e.split[i] > kPassMaxLog2) return;
+            if (!pass_cols_ok(e.split[i], e.cols[i])) return;
+            sum += e.split[i];
+        }
+        if (sum != log2n) return;
+        e.kind = TuneEntry::kMulti;
+    } else {
+        return;
+    }
+    g_table[log2n][regime] = e;
+}
+
+void load_table() {
+    FILE *f = std::fopen(table_path().c_str(), "r");
+    if (!f) return;
+    char line[512];
+    while (std::fgets(line, sizeof line, f)) {
+        if (line[0] == '#' || line[0] == '\n') continue;
+        parse_line(line);
+    }
+    std::fclose(f);
+}
+
+}  // namespace
+
+const TuneEntry &tune_lookup(unsigned log2n, int regime) {
+    std::call_once(g_once, load_table);
+    static const TuneEntry none;
+    return log2n <= (unsigned)kTuneMaxLog2 && (regime == 0 || regime == 1) ? g_table[log2n][regime] : none;
+}
+
+}  // namespace onefft

@@ -1,0 +1,168 @@
+"""Offline autotuner: measures the kernel choices for each transform size on this GPU and writes the
+tuning table the library reads at plan creation (tuning.hip).
+
+    python -m onefft_amd.tune [--min 0] [--max 30] [--out onefft_amd/tune_gfx950.tsv]
+
+* sizes up to 2^14 (one LDS pass): the four single-pass variants (nontemporal global access on or
+  off, padded or swizzled LDS);
+* larger sizes: every split n = N_1 ... N_p into 2..4 passes with factors 2^6..2^10 (4 passes only
+  from 2^24), each with the default and the widest allowed columns per workgroup;
+each in two working-set regimes, because the best choice differs between them: a batch whose
+input plus output fit the 256 MiB Infinity Cache (16 n batch = 256 MiB, or one transform), and a
+batch of 1 GiB in and 1 GiB out (one transform above 2^26 points). The library picks the regime
+per call from n and the batch (runtime.hpp: tune_regime).
+
+Each candidate is a plan created under the environment overrides (ONEFFT_BATCH_VARIANT,
+ONEFFT_PASS_SPLIT, ONEFFT_PASS_COLS) and timed with HIP events: the median of 5 launches after 2
+warm-up launches. All candidates of a size give results within FP32 rounding of each other; the
+tuner checks each against the first candidate (rel-L2 <= 3e-7 log2 n). The fastest is written.
+The table is read once per process, so this module sets ONEFFT_TUNE_FILE to a missing file before
+the first plan: the candidates are measured without any table.
+"""
+from __future__ import annotations
+
+import argparse
+import itertools
+import math
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DEFAULT_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_gfx950.tsv")
+
+
+def pass_cols_ok(log2nf: int, cols: int) -> bool:
+    """The kernels' rule (kernels.hpp: pass_cols_ok)."""
+    lp = min(log2nf, 4)
+    threads = cols << (log2nf - lp)
+    return cols in (8, 16, 32, 64) and (cols << log2nf) <= 8192 and 64 <= threads <= 1024
+
+
+def default_cols(log2nf: int) -> int:
+    for c in (16, 32, 64):
+        if pass_cols_ok(log2nf, c):
+            return c
+    return 8
+
+
+def widest_cols(log2nf: int) -> int:
+    return max(c for c in (8, 16, 32, 64) if pass_cols_ok(log2nf, c))
+
+
+def splits(L: int):
+    out = []
+    for p in (2, 3, 4):
+        if p == 4 and L < 24:
+            continue
+        for combo in itertools.product(range(6, 11), repeat=p):
+            if sum(combo) == L:
+                out.append(list(combo))
+    return out
+
+
+def candidates(L: int, single: bool):
+    if single:
+        return [dict(kind="single", params=f"nt={nt},layout={lay}", env={"ONEFFT_BATCH_VARIANT": f"{nt},{lay}"})
+                for nt in (1, 0) for lay in (0, 1)]
+    out = []
+    for s in splits(L):
+        seen = set()
+        for cols in ([default_cols(f) for f in s], [widest_cols(f) for f in s]):
+            if tuple(cols) in seen:
+                continue
+            seen.add(tuple(cols))
+            out.append(dict(kind="multi", params=f"split={','.join(map(str, s))};cols={','.join(map(str, cols))}",
+                            env={"ONEFFT_PASS_SPLIT": ",".join(map(str, s)),
+                                 "ONEFFT_PASS_COLS": ",".join(map(str, cols))}))
+    return out
+
+
+def tune_size(F, torch, timed, L: int, batch: int, single: bool) -> str:
+    """Measures every candidate for 2^L points at this batch; returns the table line of the best."""
+    n = 1 << L
+    x = torch.randn(batch, n, dtype=torch.complex64, device="cuda")
+    y = torch.empty_like(x)
+    first = None
+    results = []
+    for cand in candidates(L, single):
+        os.environ.update(cand["env"])
+        plan = F.Plan(n)
+        ms = timed(plan, x, y, batch, n)
+        if first is None:
+            first = y.clone()
+        else:
+            err = (torch.linalg.vector_norm(y - first) / torch.linalg.vector_norm(first)).item()
+            if err > 3e-7 * max(L, 1):
+                raise SystemExit(f"log2n={L} {cand['params']}: result differs from the first candidate ({err:.3g})")
+        results.append((16.0 * n * batch / (ms * 1e-3) / 1e9, cand["kind"], cand["params"]))
+        plan.close()
+        for k in cand["env"]:
+            os.environ.pop(k, None)
+    best = max(results)
+    print(f"log2n={L:2d} batch={batch:6d} best {best[1]} {best[2]} {best[0]:.0f} GB/s ({len(results)} candidates)",
+          flush=True)
+    return f"{L}\t{best[1]}\t{best[2]}\t{best[0]:.0f}\t{batch}"
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--min", type=int, default=0)
+    ap.add_argument("--max", type=int, default=30)
+    ap.add_argument("--out", default=DEFAULT_OUT)
+    args = ap.parse_args(argv)
+    os.environ["ONEFFT_TUNE_FILE"] = os.path.join(ROOT, "no-tuning-table-while-tuning")
+    for k in ("ONEFFT_BATCH_VARIANT", "ONEFFT_PASS_SPLIT", "ONEFFT_PASS_COLS"):
+        os.environ.pop(k, None)
+
+    import torch
+
+    sys.path.insert(0, ROOT)
+    import onefft_amd as F
+
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+
+    def timed(plan, x, y, batch, n):
+        run = lambda: plan.batch_ptr(x.data_ptr(), y.data_ptr(), batch, n, n, False, sp)
+        for _ in range(2):
+            run()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(5):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            run()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return statistics.median(ts)
+
+    lines = []
+    props = torch.cuda.get_device_properties(0)
+    for L in range(args.min, args.max + 1):
+        n = 1 << L
+        single = L <= 14
+        for regime in (0, 1):
+            if regime == 0:
+                if 16 * n > (256 << 20):
+                    continue
+                batch = max(1, (1 << 24) // n)
+            else:
+                batch = max(1, (1 << 26 if single else 1 << 27) // n)
+                if 16 * n * batch <= (256 << 20):
+                    continue
+            lines.append(tune_size(F, torch, timed, L, batch, single))
+            torch.cuda.empty_cache()
+    header = [
+        "# OneFFT gfx950 tuning table: log2n, kind, params, measured GB/s (16 n batch / time), batch",
+        f"# written by python -m onefft_amd.tune on {props.name} ({time.strftime('%Y-%m-%d')})",
+    ]
+    with open(args.out, "w") as f:
+        f.write("\n".join(header + lines) + "\n")
+    print(f"wrote {args.out}", flush=True)
+
+
+if __name__ == "__main__":
+    main()

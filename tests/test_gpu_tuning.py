@@ -1,0 +1,67 @@
+"""The tuning table: a plan created under a table takes its choices, results stay within the
+parity bars for every choice, and malformed lines are ignored (each case in a fresh process,
+because the table is read once per process)."""
+import os
+import subprocess
+import sys
+import textwrap
+
+import pytest
+
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+CHILD = textwrap.dedent("""
+    import sys
+    sys.path.insert(0, {root!r})
+    import numpy as np, torch
+    import onefft_amd as F
+    from oracle import oracle as O
+    s = torch.cuda.current_stream().cuda_stream
+    for L in (12, 16, 22):
+        n = 1 << L
+        x = O.normal(L, n)
+        d = torch.from_numpy(x).cuda(); o = torch.empty_like(d)
+        F.Plan(n).batch_ptr(d.data_ptr(), o.data_ptr(), 1, n, n, False, s)
+        torch.cuda.synchronize()
+        e = O.rel_rms(o.cpu().numpy(), O.fft_f64(x, n))
+        assert e < 4e-7 + 1e-7 * L, (L, e)
+    print("ok")
+""")
+
+
+def run_with_table(tmp_path, text):
+    table = tmp_path / "table.tsv"
+    table.write_text(text)
+    env = dict(os.environ, ONEFFT_TUNE_FILE=str(table))
+    for k in ("ONEFFT_BATCH_VARIANT", "ONEFFT_PASS_SPLIT", "ONEFFT_PASS_COLS"):
+        env.pop(k, None)
+    r = subprocess.run([sys.executable, "-c", CHILD.format(root=ROOT)], env=env, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0 and "ok" in r.stdout, r.stdout + r.stderr
+
+
+def test_table_choices_are_used_and_correct(tmp_path):
+    run_with_table(tmp_path, "# test table\n"
+                             "12\tsingle\tnt=0,layout=1\t1\t1\n"
+                             "16\tmulti\tsplit=6,10;cols=64,8\t1\t1\n"
+                             "22\tmulti\tsplit=6,6,10;cols=32,16,8\t1\t1\n")
+
+
+def test_malformed_table_is_ignored(tmp_path):
+    run_with_table(tmp_path, "12 single garbage\n"
+                             "16\tmulti\tsplit=6,9;cols=16,16\t1\t1\n"      # factors do not sum to 16
+                             "22\tmulti\tsplit=11,11;cols=8,8\t1\t1\n"      # factor above 2^10
+                             "99\tsingle\tnt=1,layout=0\t1\t1\n")
+
+
+def test_committed_table_parses():
+    path = os.path.join(ROOT, "onefft_amd", "tune_gfx950.tsv")
+    if not os.path.exists(path):
+        pytest.skip("no committed tuning table")
+    for line in open(path):
+        if line.startswith("#") or not line.strip():
+            continue
+        log2n, kind, params, gbs, batch = line.rstrip("\n").split("\t")
+        assert 0 <= int(log2n) <= 30 and kind in ("single", "multi") and float(gbs) > 0 and int(batch) >= 1

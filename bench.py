@@ -61,10 +61,11 @@ def time_ms(torch, stream, fn, reps):
 
 
 def variants_ab(torch, F, sizes):
-    """Interleaved A/B of the single-pass kernel variants (ONEFFT_BATCH_VARIANT = "nt,layout")."""
+    """Interleaved A/B of the single-pass kernel variants (ONEFFT_BATCH_VARIANT = "nt,layout,tw64")."""
     import statistics
 
-    variants = {"padded": "0,0", "padded_nt": "1,0", "swizzled": "0,1", "swizzled_nt": "1,1"}
+    variants = {"padded": "0,0,0", "padded_nt": "1,0,0", "swizzled": "0,1,0", "swizzled_nt": "1,1,0",
+                "padded_tw64": "0,0,1", "padded_nt_tw64": "1,0,1"}
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
     old = os.environ.get("ONEFFT_BATCH_VARIANT")
@@ -77,19 +78,22 @@ def variants_ab(torch, F, sizes):
             os.environ["ONEFFT_BATCH_VARIANT"] = v
             plans[k] = F.Plan(n)
         first = None
+        bitwise = {}
         for k, p in plans.items():
             p.batch_ptr(x.data_ptr(), y.data_ptr(), batch, n, n, False, sp)
             torch.cuda.synchronize()
             if first is None:
                 first = y.clone()
-            elif not torch.equal(first, y):
-                raise SystemExit(f"variant {k} differs from the first variant at log2 n = {L}")
+            err = (torch.linalg.vector_norm(y - first) / torch.linalg.vector_norm(first)).item()
+            if err > 3e-7 * L:
+                raise SystemExit(f"variant {k} differs from the first variant at log2 n = {L}: rel-L2 {err:.3g}")
+            bitwise[k] = bool(torch.equal(first, y))
         times = {k: [] for k in plans}
         for _ in range(6):
             for k, p in plans.items():
                 times[k].append(time_ms(torch, stream, lambda: p.batch_ptr(x.data_ptr(), y.data_ptr(), batch, n, n,
                                                                            False, sp), 10))
-        row = {"log2n": L, "batch": batch, "identical_outputs": True}
+        row = {"log2n": L, "batch": batch, "bitwise_equal_to_first": bitwise}
         for k, v in times.items():
             row[k] = {"min_us": round(min(v) * 1e3, 1), "median_us": round(statistics.median(v) * 1e3, 1),
                       "gb_s_at_min": round(16.0 * n * batch / (min(v) * 1e-3) / 1e9)}

@@ -21,16 +21,45 @@ namespace onefft {
 // Thread j of the transform (0 <= j < 2^LOG2N / P) works on src/dst; `live` false skips the global
 // loads and stores but keeps the thread in every barrier. L is the transform's LDS region of
 // lds_extent<LAYOUT>(2^LOG2N) values (unused when the transform fits one thread). NT: nontemporal
-// global loads and stores.
-template <int LOG2N, bool INV, bool NT = false, int LAYOUT = kPadded>
-__device__ __forceinline__ void single_body(const cf *__restrict__ src, cf *__restrict__ dst, bool live,
-                                            const cf *__restrict__ tw, cf *L, uint32_t j) {
-    constexpr int LOG2P = batch_log2p(LOG2N);
-    constexpr int N = 1 << LOG2N;
-    constexpr int P = 1 << LOG2P;
-    constexpr uint32_t T = N / P;
-    constexpr int S = stage_count(LOG2N, LOG2P);
-    cf v[P];
+// global loads and stores. The body is split into its global loads (single_load) and the rest
+// (single_finish), so a pipelined kernel can load the next transform while finishing this one.
+
+template <int LOG2N>
+struct Single {
+    static constexpr int LOG2P = batch_log2p(LOG2N);
+    static constexpr int N = 1 << LOG2N;
+    static constexpr int P = 1 << LOG2P;
+    static constexpr uint32_t T = N / P;
+    static constexpr int S = stage_count(LOG2N, LOG2P);
+};
+
+// stage 0's inputs: element j + T q + r N / R_0 into v[q R_0 + r]
+template <int LOG2N, bool INV, bool NT>
+__device__ __forceinline__ void single_load(const cf *__restrict__ src, bool live, uint32_t j,
+                                            cf (&v)[Single<LOG2N>::P]) {
+    using G = Single<LOG2N>;
+    constexpr int R = 1 << stage_log2r(LOG2N, G::LOG2P, 0);
+    constexpr int BF = G::P / R;
+    constexpr uint32_t STRIDE = G::N / R;
+    if (live) {
+        static_for<0, BF>([&](auto q) {
+            static_for<0, R>([&](auto r) {
+                constexpr int qq = decltype(q)::value, rr = decltype(r)::value;
+                v[qq * R + rr] = maybe_swap<INV>(gload<NT>(&src[j + G::T * qq + rr * STRIDE]));
+            });
+        });
+    } else {
+        static_for<0, G::P>([&](auto i) { v[i] = mk(0.f, 0.f); });
+    }
+}
+
+template <int LOG2N, bool INV, bool NT, int LAYOUT, bool TW64 = false>
+__device__ __forceinline__ void single_finish(cf (&v)[Single<LOG2N>::P], cf *__restrict__ dst, bool live,
+                                              const cf *__restrict__ tw, cf *L, uint32_t j,
+                                              const cd *__restrict__ tw64 = nullptr) {
+    using G = Single<LOG2N>;
+    constexpr int LOG2P = G::LOG2P, N = G::N, P = G::P, S = G::S;
+    constexpr uint32_t T = G::T;
 
     static_for<0, S>([&](auto sI) {
         constexpr int s = decltype(sI)::value;
@@ -41,19 +70,10 @@ __device__ __forceinline__ void single_body(const cf *__restrict__ src, cf *__re
         constexpr int BF = P / R;  // butterflies per thread
         constexpr uint32_t STRIDE = N / R;
         constexpr uint32_t TWO = stage_tw_offset(LOG2N, LOG2P, s);
+        constexpr bool BASE64 = TW64 && stage_uses_tw64(LOG2N, LOG2P, s);
+        constexpr uint32_t TWO64 = stage_tw64_offset(LOG2N, LOG2P, s);
 
-        if constexpr (s == 0) {
-            if (live) {
-                static_for<0, BF>([&](auto q) {
-                    static_for<0, R>([&](auto r) {
-                        constexpr int qq = decltype(q)::value, rr = decltype(r)::value;
-                        v[qq * R + rr] = maybe_swap<INV>(gload<NT>(&src[j + T * qq + rr * STRIDE]));
-                    });
-                });
-            } else {
-                static_for<0, P>([&](auto i) { v[i] = mk(0.f, 0.f); });
-            }
-        } else {
+        if constexpr (s > 0) {
             static_for<0, BF>([&](auto q) {
                 static_for<0, R>([&](auto r) {
                     constexpr int qq = decltype(q)::value, rr = decltype(r)::value;
@@ -66,10 +86,21 @@ __device__ __forceinline__ void single_body(const cf *__restrict__ src, cf *__re
             constexpr int qq = decltype(q)::value;
             if constexpr (s > 0) {
                 const uint32_t k = (j + T * qq) & (NS - 1);
-                static_for<1, R>([&](auto r) {
-                    constexpr int rr = decltype(r)::value;
-                    v[qq * R + rr] = mul(v[qq * R + rr], tw[TWO + (rr - 1) * NS + k]);
-                });
+                if constexpr (BASE64) {
+                    // w^(r k) = w^k ... w^k in FP64, rounded once to FP32
+                    const cd base = tw64[TWO64 + k];
+                    cd w = base;
+                    static_for<1, R>([&](auto r) {
+                        constexpr int rr = decltype(r)::value;
+                        v[qq * R + rr] = mul(v[qq * R + rr], round32(w));
+                        if constexpr (rr + 1 < R) w = mul64(w, base);
+                    });
+                } else {
+                    static_for<1, R>([&](auto r) {
+                        constexpr int rr = decltype(r)::value;
+                        v[qq * R + rr] = mul(v[qq * R + rr], tw[TWO + (rr - 1) * NS + k]);
+                    });
+                }
             }
             dft<R>(&v[qq * R]);
         });
@@ -98,6 +129,15 @@ __device__ __forceinline__ void single_body(const cf *__restrict__ src, cf *__re
             __syncthreads();
         }
     });
+}
+
+template <int LOG2N, bool INV, bool NT = false, int LAYOUT = kPadded, bool TW64 = false>
+__device__ __forceinline__ void single_body(const cf *__restrict__ src, cf *__restrict__ dst, bool live,
+                                            const cf *__restrict__ tw, cf *L, uint32_t j,
+                                            const cd *__restrict__ tw64 = nullptr) {
+    cf v[Single<LOG2N>::P];
+    single_load<LOG2N, INV, NT>(src, live, j, v);
+    single_finish<LOG2N, INV, NT, LAYOUT, TW64>(v, dst, live, tw, L, j, tw64);
 }
 
 // ---- one tile of one pass of a multi-pass transform -------------------------------------------

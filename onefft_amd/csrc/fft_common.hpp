@@ -196,6 +196,23 @@ __host__ __device__ constexpr uint32_t stage_tw_size(int log2n, int log2p) {
     return stage_tw_offset(log2n, log2p, stage_count(log2n, log2p));
 }
 
+// Stages whose butterflies read distinct twiddles in every lane (Ns >= 64) can instead take one
+// FP64 base w_{Ns R}^k per butterfly and form w^(r k), r = 2..R-1, by an FP64 recurrence, each
+// rounded once to FP32: 16 bytes of table per butterfly instead of 8 (R - 1). The FP64 base table
+// of stage s holds w_{Ns R}^k for k < Ns at stage_tw64_offset(.., s).
+__host__ __device__ constexpr bool stage_uses_tw64(int log2n, int log2p, int s) {
+    return s > 0 && stage_log2ns(log2n, log2p, s) >= 6 && stage_log2r(log2n, log2p, s) >= 2;
+}
+__host__ __device__ constexpr uint32_t stage_tw64_offset(int log2n, int log2p, int s) {
+    uint32_t off = 0;
+    for (int i = 1; i < s; i++)
+        if (stage_uses_tw64(log2n, log2p, i)) off += 1u << stage_log2ns(log2n, log2p, i);
+    return off;
+}
+__host__ __device__ constexpr uint32_t stage_tw64_size(int log2n, int log2p) {
+    return stage_tw64_offset(log2n, log2p, stage_count(log2n, log2p));
+}
+
 // ---- LDS layout ---------------------------------------------------------------------------
 //
 // Two layouts of a transform's values in LDS, chosen per kernel variant:

@@ -11,9 +11,10 @@
 
 namespace onefft {
 
-template <int LOG2N, bool INV, bool NT, int LAYOUT>
+template <int LOG2N, bool INV, bool NT, int LAYOUT, bool TW64>
 __global__ __launch_bounds__(batch_wg(LOG2N)) void fft_batch_kernel(const cf *__restrict__ in, cf *__restrict__ out,
-                                                                      const cf *__restrict__ tw, uint64_t in_dist,
+                                                                      const cf *__restrict__ tw,
+                                                                      const cd *__restrict__ tw64, uint64_t in_dist,
                                                                       uint64_t out_dist, uint64_t batch) {
     constexpr int N = 1 << LOG2N;
     constexpr uint32_t T = N >> batch_log2p(LOG2N);  // threads per transform
@@ -26,18 +27,18 @@ __global__ __launch_bounds__(batch_wg(LOG2N)) void fft_batch_kernel(const cf *__
     const uint32_t t = threadIdx.x;
     const uint32_t c = t / T;
     const uint64_t which = (uint64_t)blockIdx.x * PER + c;
-    single_body<LOG2N, INV, NT, LAYOUT>(in + which * in_dist, out + which * out_dist, which < batch, tw,
-                                        lds + (USES_LDS ? c * SPAN : 0), t % T);
+    single_body<LOG2N, INV, NT, LAYOUT, TW64>(in + which * in_dist, out + which * out_dist, which < batch, tw,
+                                              lds + (USES_LDS ? c * SPAN : 0), t % T, tw64);
 }
 
-template <int LOG2N, bool NT, int LAYOUT>
+template <int LOG2N, bool NT, int LAYOUT, bool TW64 = false>
 static void launch_variant(bool inverse, const BatchArgs &a, dim3 grid, hipStream_t stream) {
     if (inverse)
-        hipLaunchKernelGGL((fft_batch_kernel<LOG2N, true, NT, LAYOUT>), grid, dim3(batch_wg(LOG2N)), 0, stream, a.in,
-                           a.out, a.tw, a.in_dist, a.out_dist, a.batch);
+        hipLaunchKernelGGL((fft_batch_kernel<LOG2N, true, NT, LAYOUT, TW64>), grid, dim3(batch_wg(LOG2N)), 0, stream,
+                           a.in, a.out, a.tw, a.tw64, a.in_dist, a.out_dist, a.batch);
     else
-        hipLaunchKernelGGL((fft_batch_kernel<LOG2N, false, NT, LAYOUT>), grid, dim3(batch_wg(LOG2N)), 0, stream, a.in,
-                           a.out, a.tw, a.in_dist, a.out_dist, a.batch);
+        hipLaunchKernelGGL((fft_batch_kernel<LOG2N, false, NT, LAYOUT, TW64>), grid, dim3(batch_wg(LOG2N)), 0, stream,
+                           a.in, a.out, a.tw, a.tw64, a.in_dist, a.out_dist, a.batch);
 }
 
 template <int LOG2N>
@@ -48,6 +49,14 @@ static hipError_t launch_batch_n(bool inverse, const BatchArgs &a, BatchVariant 
     if (blocks > 0x7fffffffu) return hipErrorInvalidValue;
     const dim3 grid((uint32_t)blocks);
     constexpr bool USES_LDS = stage_count(LOG2N, batch_log2p(LOG2N)) > 1;
+    constexpr bool HAS_TW64 = stage_tw64_size(LOG2N, batch_log2p(LOG2N)) > 0;
+    if constexpr (HAS_TW64) {
+        if (v.tw64) {  // padded layout only
+            if (v.nt) launch_variant<LOG2N, true, kPadded, true>(inverse, a, grid, stream);
+            else launch_variant<LOG2N, false, kPadded, true>(inverse, a, grid, stream);
+            return hipGetLastError();
+        }
+    }
     const bool swz = USES_LDS && v.layout == kSwizzled;
     if (v.nt) {
         if (swz) launch_variant<LOG2N, true, kSwizzled>(inverse, a, grid, stream);

@@ -28,16 +28,20 @@ struct BatchArgs {
     const cf *in;
     cf *out;
     const cf *tw;        // stage twiddles of this size (stage_tw_size(log2n, batch_log2p(log2n)) entries)
+    const cd *tw64;      // FP64 bases of the wide stages (stage_tw64_size entries)
     uint64_t in_dist;    // complex values between transforms
     uint64_t out_dist;
     uint64_t batch;
 };
 
-// Kernel variant of the single-pass transform: nontemporal global access, and the LDS layout
-// (fft_common.hpp: kPadded / kSwizzled). The twiddle table does not depend on the variant.
+// Kernel variant of the single-pass transform: nontemporal global access, the LDS layout
+// (fft_common.hpp: kPadded / kSwizzled), and FP64-base twiddles for the wide stages
+// (fft_common.hpp: stage_uses_tw64; with the padded layout only). The plan's tables serve every
+// variant.
 struct BatchVariant {
     bool nt = true;
     int layout = 0;
+    bool tw64 = true;
 };
 
 // One launch: `batch` transforms of 2^log2n points, 0 <= log2n <= kMaxSinglePassLog2.

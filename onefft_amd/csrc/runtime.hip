@@ -92,6 +92,18 @@ using onefft::fill_stage_table;
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+// FP64 bases w_{Ns R}^k, k < Ns, of the stages that use them (fft_common.hpp: stage_uses_tw64).
+void fill_stage_table64(int log2n, int log2p, cd *table) {
+    const int stages = onefft::stage_count(log2n, log2p);
+    for (int s = 1; s < stages; s++) {
+        if (!onefft::stage_uses_tw64(log2n, log2p, s)) continue;
+        const uint32_t off = onefft::stage_tw64_offset(log2n, log2p, s);
+        const int lr = onefft::stage_log2r(log2n, log2p, s);
+        const int lns = onefft::stage_log2ns(log2n, log2p, s);
+        for (uint32_t k = 0; k < (1u << lns); k++) root64(k, lns + lr, &table[off + k].r, &table[off + k].i);
+    }
+}
+
 // Pass split of a multi-pass size: log2 N_1 .. log2 N_p (N_1 is transformed last), and the
 // columns per workgroup of each factor's pass. The environment variable ONEFFT_PASS_SPLIT
 // ("a,b[,c[,d]]", log2 N_1 first) overrides for sizes whose factors it matches, to sweep splits
@@ -165,10 +177,11 @@ onefft::BatchVariant choose_variant(unsigned log2n, int regime) {
     const onefft::TuneEntry &t = onefft::tune_lookup(log2n, regime);
     if (t.kind == onefft::TuneEntry::kSingle) v = t.variant;
     if (const char *env = std::getenv("ONEFFT_BATCH_VARIANT")) {
-        int nt = 1, layout = 0;
-        if (std::sscanf(env, "%d,%d", &nt, &layout) == 2) {
+        int nt = 1, layout = 0, tw64 = 1;
+        if (std::sscanf(env, "%d,%d,%d", &nt, &layout, &tw64) >= 2) {
             v.nt = nt != 0;
             v.layout = layout ? 1 : 0;
+            v.tw64 = tw64 != 0;
         }
     }
     return v;
@@ -201,6 +214,7 @@ struct onefft_plan {
     void *dmem = nullptr;  // the one device allocation holding every table
     bool multi = false;    // false: one LDS pass
     const cf *tw_single = nullptr;
+    const cd *tw64_single = nullptr;
     BatchVariant variant[2];  // single-pass kernel variant per regime
     PassConfig cfg[2];        // multi-pass split per regime
 };
@@ -251,7 +265,7 @@ int execute_device(const onefft_plan *plan, const cf *d_in, cf *d_out, uint64_t 
     if (batch == 0) return 0;
     const int regime = tune_regime(plan->n, batch);
     if (!plan->multi) {
-        BatchArgs a{d_in, d_out, plan->tw_single, in_dist, out_dist, batch};
+        BatchArgs a{d_in, d_out, plan->tw_single, plan->tw64_single, in_dist, out_dist, batch};
         hipError_t e = launch_batch((int)plan->log2n, inverse, a, plan->variant[regime], stream);
         if (e != hipSuccess) {
             set_error("single-pass launch", e);
@@ -307,12 +321,14 @@ extern "C" onefft_plan *onefft_plan_create(size_t n) {
         image.resize(off + bytes);
         return off;
     };
-    size_t off_single = 0, off_stage[2][kMaxPasses] = {}, off_lo = 0, off_hi = 0;
+    size_t off_single = 0, off_single64 = 0, off_stage[2][kMaxPasses] = {}, off_lo = 0, off_hi = 0;
     if (log2n <= (unsigned)kMaxSinglePassLog2) {
         const int lp = batch_log2p((int)log2n);
         const uint32_t entries = stage_tw_size((int)log2n, lp);
         off_single = reserve(entries * sizeof(cf) + sizeof(cf));
         fill_stage_table((int)log2n, lp, reinterpret_cast<cf *>(image.data() + off_single));
+        off_single64 = reserve(stage_tw64_size((int)log2n, lp) * sizeof(cd) + sizeof(cd));
+        fill_stage_table64((int)log2n, lp, reinterpret_cast<cd *>(image.data() + off_single64));
     } else {
         plan->multi = true;
         // two-level FP64 table of w_n, shared by both configurations
@@ -378,6 +394,7 @@ extern "C" onefft_plan *onefft_plan_create(size_t n) {
     auto dev = [&](size_t off) { return reinterpret_cast<unsigned char *>(plan->dmem) + off; };
     if (!plan->multi) {
         plan->tw_single = reinterpret_cast<const cf *>(dev(off_single));
+        plan->tw64_single = reinterpret_cast<const cd *>(dev(off_single64));
     } else {
         for (int r = 0; r < 2; r++) {
             for (int i = 0; i < plan->cfg[r].passes; i++) {

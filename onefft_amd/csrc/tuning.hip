@@ -2,7 +2,7 @@
 // multi-pass split and columns per workgroup that measured fastest on this GPU.
 //
 // File: tab-separated lines "log2n  kind  params  gb_s  batch", '#' comments, where kind is
-// "single" (params "nt=0|1,layout=0|1") or "multi" (params "split=a,b[,c[,d]];cols=x,y[,..]",
+// "single" (params "nt=0|1,layout=0|1[,tw64=0|1]") or "multi" (params "split=a,b[,c[,d]];cols=x,y[,..]",
 // log2 N_1 first). The batch the line was measured at sets its working-set regime
 // (runtime.hpp: tune_regime); a size may have a line per regime. Written by `python -m onefft_amd.tune`; read once, at the first plan creation,
 // from $ONEFFT_TUNE_FILE or tune_gfx950.tsv next to the library. Plan creation stays O(n) with no
@@ -61,11 +61,12 @@ void parse_line(const char *line) {
     const int regime = tune_regime(uint64_t(1) << log2n, batch);
     TuneEntry e;
     if (!std::strcmp(kind, "single")) {
-        int nt = 1, layout = 0;
-        if (std::sscanf(params, "nt=%d,layout=%d", &nt, &layout) != 2) return;
+        int nt = 1, layout = 0, tw64 = 0;  // tables written before FP64 bases existed meant tw64=0
+        if (std::sscanf(params, "nt=%d,layout=%d,tw64=%d", &nt, &layout, &tw64) < 2) return;
         e.kind = TuneEntry::kSingle;
         e.variant.nt = nt != 0;
         e.variant.layout = layout ? 1 : 0;
+        e.variant.tw64 = tw64 != 0;
     } else if (!std::strcmp(kind, "multi")) {
         const char *sp = std::strstr(params, "split="), *cp = std::strstr(params, "cols=");
         if (!sp || !cp) return;

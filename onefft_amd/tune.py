@@ -3,8 +3,8 @@ tuning table the library reads at plan creation (tuning.hip).
 
     python -m onefft_amd.tune [--min 0] [--max 30] [--out onefft_amd/tune_gfx950.tsv]
 
-* sizes up to 2^14 (one LDS pass): the four single-pass variants (nontemporal global access on or
-  off, padded or swizzled LDS);
+* sizes up to 2^14 (one LDS pass): the single-pass variants (nontemporal global access on or off,
+  padded or swizzled LDS, and from 2^10 FP64-base twiddles for the wide stages);
 * larger sizes: every split n = N_1 ... N_p into 2..4 passes with factors 2^6..2^10 (4 passes only
   from 2^24), each with the default and the widest allowed columns per workgroup;
 each in two working-set regimes, because the best choice differs between them: a batch whose
@@ -64,8 +64,11 @@ def splits(L: int):
 
 def candidates(L: int, single: bool):
     if single:
-        return [dict(kind="single", params=f"nt={nt},layout={lay}", env={"ONEFFT_BATCH_VARIANT": f"{nt},{lay}"})
-                for nt in (1, 0) for lay in (0, 1)]
+        combos = [(nt, lay, 0) for nt in (1, 0) for lay in (0, 1)]
+        if L >= 10:  # sizes with a stage wide enough for FP64-base twiddles (padded layout only)
+            combos += [(nt, 0, 1) for nt in (1, 0)]
+        return [dict(kind="single", params=f"nt={nt},layout={lay},tw64={t64}",
+                     env={"ONEFFT_BATCH_VARIANT": f"{nt},{lay},{t64}"}) for nt, lay, t64 in combos]
     out = []
     for s in splits(L):
         seen = set()

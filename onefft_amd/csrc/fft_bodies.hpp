@@ -235,10 +235,21 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, const cf *__restric
             constexpr int qq = decltype(q)::value;
             if constexpr (s > 0) {
                 const uint32_t k = (j + T * qq) & (NS - 1);
-                static_for<1, R>([&](auto r) {
-                    constexpr int rr = decltype(r)::value;
-                    v[qq * R + rr] = mul(v[qq * R + rr], a.tw[TWO + (rr - 1) * NS + k]);
-                });
+                if constexpr (stage_uses_tw64(LOG2NF, LOG2P, s)) {
+                    // w^(r k) by an FP64 recurrence from the base w^k, each rounded once
+                    const cd base = a.tw64[stage_tw64_offset(LOG2NF, LOG2P, s) + k];
+                    cd w = base;
+                    static_for<1, R>([&](auto r) {
+                        constexpr int rr = decltype(r)::value;
+                        v[qq * R + rr] = mul(v[qq * R + rr], round32(w));
+                        if constexpr (rr + 1 < R) w = mul64(w, base);
+                    });
+                } else {
+                    static_for<1, R>([&](auto r) {
+                        constexpr int rr = decltype(r)::value;
+                        v[qq * R + rr] = mul(v[qq * R + rr], a.tw[TWO + (rr - 1) * NS + k]);
+                    });
+                }
             }
             dft<R>(&v[qq * R]);
         });

@@ -89,7 +89,7 @@ __device__ __forceinline__ void first_stage(const Column &io, uint32_t N, uint32
     (void)v;
 }
 
-__device__ void column_fft(const Column &io, uint32_t log2n, const cf *tw, cf *L, uint32_t j) {
+__device__ void column_fft(const Column &io, uint32_t log2n, const cf *tw, const cd *tw64, cf *L, uint32_t j) {
     const uint32_t N = 1u << log2n;
     const uint32_t P = N < 16 ? N : 16;
     const uint32_t r0_log2 = log2n <= 4 ? log2n : (log2n % 4 ? log2n % 4 : 4);
@@ -106,16 +106,27 @@ __device__ void column_fft(const Column &io, uint32_t log2n, const cf *tw, cf *L
     __syncthreads();
     // radix-16 stages: butterfly j reads j + r N/16, twiddles w_{16 NS}^(r (j mod NS))
     const uint32_t stride = N / 16;
-    uint32_t log2ns = r0_log2, off = 0;
+    uint32_t log2ns = r0_log2, off = 0, off64 = 0;
     for (;;) {
         const uint32_t NS = 1u << log2ns;
         const bool last = log2ns + 4 == log2n;
         static_for<0, 16>([&](auto r) { v[r] = L[lds_pad(j + decltype(r)::value * stride)]; });
         const uint32_t k = j & (NS - 1);
-        static_for<1, 16>([&](auto r) {
-            constexpr int rr = decltype(r)::value;
-            v[rr] = mul(v[rr], tw[off + (rr - 1) * NS + k]);
-        });
+        if (log2ns >= 6) {
+            // wide stage: w^(r k) by an FP64 recurrence from the base w^k, each rounded once
+            const cd base = tw64[off64 + k];
+            cd w = base;
+            static_for<1, 16>([&](auto r) {
+                v[r] = mul(v[r], round32(w));
+                if constexpr (decltype(r)::value + 1 < 16) w = mul64(w, base);
+            });
+            off64 += NS;
+        } else {
+            static_for<1, 16>([&](auto r) {
+                constexpr int rr = decltype(r)::value;
+                v[rr] = mul(v[rr], tw[off + (rr - 1) * NS + k]);
+            });
+        }
         dft<16>(v);
         if (last) {
             if (io.live) static_for<0, 16>([&](auto m) {
@@ -146,12 +157,12 @@ __device__ void single_task(const RaggedArgs &a, const RaggedTask &tk, uint32_t 
     io.in_step = io.out_step = 1;
     io.swap_in = io.swap_out = inv;
     io.twiddle = false;
-    column_fft(io, log2n, a.tw32 + tk.tw, lds + c * lds_span(N), t % T);
+    column_fft(io, log2n, a.tw32 + tk.tw, a.tw64 + a.size[log2n].tw64, lds + c * lds_span(N), t % T);
 }
 
 // one tile (C = 4096 / N_f columns) of one pass of a two-pass item (index scheme of k_pass.hip)
-__device__ void pass_tile(const RaggedArgs &a, const RaggedSize &z, const cf *in, cf *out, uint32_t tw, bool first,
-                          uint32_t tile, bool inv, cf *lds, uint32_t t) {
+__device__ void pass_tile(const RaggedArgs &a, const RaggedSize &z, const cf *in, cf *out, uint32_t tw, uint32_t tw64,
+                          bool first, uint32_t tile, bool inv, cf *lds, uint32_t t) {
     const uint32_t lf = first ? z.l2 : z.l1;     // this pass's sub-transform
     const uint32_t C = kRaggedTaskPoints >> lf;  // columns per tile
     const uint32_t c = t % C, j = t / C;
@@ -181,7 +192,7 @@ __device__ void pass_tile(const RaggedArgs &a, const RaggedSize &z, const cf *in
         io.swap_out = inv;
         io.low = col;
     }
-    column_fft(io, lf, a.tw32 + tw, lds + c * pass_span((int)lf), j);
+    column_fft(io, lf, a.tw32 + tw, a.tw64 + tw64, lds + c * pass_span((int)lf), j);
 }
 
 // a whole two-pass item: every first-pass tile, a barrier, every second-pass tile
@@ -194,16 +205,16 @@ __device__ void two_pass_item(const RaggedArgs &a, const RaggedTask &tk, uint32_
     // as new values and reloads the twiddles per tile instead of holding every tile's twiddles in
     // registers across the loop (which would cost occupancy).
     for (uint32_t tile = 0; tile < tiles; tile++) {
-        uint32_t tw = tk.tw;
-        asm volatile("" : "+v"(tw));
-        pass_tile(a, z, a.in + off, a.out + off, tw, true, tile, inv, lds, t);
+        uint32_t tw = tk.tw, tw64 = z.tw64_2;
+        asm volatile("" : "+v"(tw), "+v"(tw64));
+        pass_tile(a, z, a.in + off, a.out + off, tw, tw64, true, tile, inv, lds, t);
         __syncthreads();  // LDS reuse
     }
     // the first pass's global stores, made by this workgroup, are complete and visible to it
     for (uint32_t tile = 0; tile < tiles; tile++) {
-        uint32_t tw = tk.arg;
-        asm volatile("" : "+v"(tw));
-        pass_tile(a, z, a.in + off, a.out + off, tw, false, tile, inv, lds, t);
+        uint32_t tw = tk.arg, tw64 = z.tw64_1;
+        asm volatile("" : "+v"(tw), "+v"(tw64));
+        pass_tile(a, z, a.in + off, a.out + off, tw, tw64, false, tile, inv, lds, t);
         __syncthreads();
     }
 }

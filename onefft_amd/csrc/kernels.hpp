@@ -71,6 +71,7 @@ struct PassArgs {
     const cf *in;
     cf *out;
     const cf *tw;         // stage twiddles of the sub-transform size
+    const cd *tw64;       // FP64 bases of its wide stages (fft_common.hpp: stage_uses_tw64)
     const cd *tw_lo;      // w_n^i, i < 2^lo_bits (FP64)
     const cd *tw_hi;      // w_n^(i 2^lo_bits), i < n / 2^lo_bits (FP64)
     uint64_t in_dist;     // complex values between transforms of a batch
@@ -115,8 +116,10 @@ struct RaggedTask {
 
 struct RaggedSize {
     uint32_t tw;       // single-pass sizes: offset of the stage table in the FP32 table blob
+    uint32_t tw64;     // ... and of its FP64 stage bases in the FP64 blob
     uint32_t l1, l2;   // two-pass sizes: log2 N_1 (transformed last) and log2 N_2
     uint32_t tw1, tw2; // stage tables of N_1 and N_2
+    uint32_t tw64_1, tw64_2;  // FP64 stage bases of N_1 and N_2
     uint32_t lo, hi;   // offsets of the FP64 twiddle tables of w_n in the FP64 blob
     uint32_t lo_bits;
 };

@@ -101,6 +101,18 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
         }
         return (uint32_t)stage_at[log2];
     };
+    int32_t stage64_at[kRaggedMaxLog2 + 1];
+    for (auto &v : stage64_at) v = -1;
+    auto stage64_table = [&](int log2) -> uint32_t {
+        if (stage64_at[log2] < 0) {
+            const int lp = batch_log2p(log2);
+            const uint32_t entries = stage_tw64_size(log2, lp);
+            stage64_at[log2] = (int32_t)tw64.size();
+            tw64.resize(tw64.size() + entries + 1);
+            if (entries) fill_stage_table64(log2, lp, tw64.data() + stage64_at[log2]);
+        }
+        return (uint32_t)stage64_at[log2];
+    };
     RaggedArgs &a = plan->args;
     std::memset(&a, 0, sizeof(a));
     for (int l = 0; l <= kRaggedMaxLog2; l++) {
@@ -108,11 +120,14 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
         RaggedSize &z = a.size[l];
         if (l <= kRaggedSingleMaxLog2) {
             z.tw = stage_table(l);
+            z.tw64 = stage64_table(l);
         } else {
             z.l1 = (uint32_t)l / 2;
             z.l2 = (uint32_t)l - z.l1;
             z.tw1 = stage_table((int)z.l1);
             z.tw2 = stage_table((int)z.l2);
+            z.tw64_1 = stage64_table((int)z.l1);
+            z.tw64_2 = stage64_table((int)z.l2);
             z.lo_bits = ((uint32_t)l + 1) / 2;
             z.lo = (uint32_t)tw64.size();
             for (uint64_t i = 0; i < (uint64_t(1) << z.lo_bits); i++) {

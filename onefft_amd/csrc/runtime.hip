@@ -83,25 +83,27 @@ void fill_stage_table(int log2n, int log2p, cf *table) {
     }
 }
 
+// FP64 bases w_{Ns R}^k, k < Ns, of the stages that use them (fft_common.hpp: stage_uses_tw64).
+void fill_stage_table64(int log2n, int log2p, cd *table) {
+    const int stages = stage_count(log2n, log2p);
+    for (int s = 1; s < stages; s++) {
+        if (!stage_uses_tw64(log2n, log2p, s)) continue;
+        const uint32_t off = stage_tw64_offset(log2n, log2p, s);
+        const int lr = stage_log2r(log2n, log2p, s);
+        const int lns = stage_log2ns(log2n, log2p, s);
+        for (uint32_t k = 0; k < (1u << lns); k++) root64(k, lns + lr, &table[off + k].r, &table[off + k].i);
+    }
+
 }  // namespace onefft
 
 namespace {
 
+using onefft::fill_stage_table64;
 using onefft::root64;
 using onefft::fill_stage_table;
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
-// FP64 bases w_{Ns R}^k, k < Ns, of the stages that use them (fft_common.hpp: stage_uses_tw64).
-void fill_stage_table64(int log2n, int log2p, cd *table) {
-    const int stages = onefft::stage_count(log2n, log2p);
-    for (int s = 1; s < stages; s++) {
-        if (!onefft::stage_uses_tw64(log2n, log2p, s)) continue;
-        const uint32_t off = onefft::stage_tw64_offset(log2n, log2p, s);
-        const int lr = onefft::stage_log2r(log2n, log2p, s);
-        const int lns = onefft::stage_log2ns(log2n, log2p, s);
-        for (uint32_t k = 0; k < (1u << lns); k++) root64(k, lns + lr, &table[off + k].r, &table[off + k].i);
-    }
 }
 
 // Pass split of a multi-pass size: log2 N_1 .. log2 N_p (N_1 is transformed last), and the
@@ -321,7 +323,8 @@ extern "C" onefft_plan *onefft_plan_create(size_t n) {
         image.resize(off + bytes);
         return off;
     };
-    size_t off_single = 0, off_single64 = 0, off_stage[2][kMaxPasses] = {}, off_lo = 0, off_hi = 0;
+    size_t off_single = 0, off_single64 = 0, off_stage[2][kMaxPasses] = {}, off_stage64[2][kMaxPasses] = {};
+    size_t off_lo = 0, off_hi = 0;
     if (log2n <= (unsigned)kMaxSinglePassLog2) {
         const int lp = batch_log2p((int)log2n);
         const uint32_t entries = stage_tw_size((int)log2n, lp);
@@ -354,6 +357,8 @@ extern "C" onefft_plan *onefft_plan_create(size_t n) {
                 const int lp = pass_log2p(lf);
                 off_stage[r][i] = reserve(stage_tw_size(lf, lp) * sizeof(cf) + sizeof(cf));
                 fill_stage_table(lf, lp, reinterpret_cast<cf *>(image.data() + off_stage[r][i]));
+                off_stage64[r][i] = reserve(stage_tw64_size(lf, lp) * sizeof(cd) + sizeof(cd));
+                fill_stage_table64(lf, lp, reinterpret_cast<cd *>(image.data() + off_stage64[r][i]));
                 PassArgs &a = cfg.pass[i].args;
                 std::memset(&a, 0, sizeof(a));
                 cfg.pass[i].log2nf = lf;
@@ -400,6 +405,7 @@ extern "C" onefft_plan *onefft_plan_create(size_t n) {
             for (int i = 0; i < plan->cfg[r].passes; i++) {
                 PassArgs &a = plan->cfg[r].pass[i].args;
                 a.tw = reinterpret_cast<const cf *>(dev(off_stage[r][i]));
+                a.tw64 = reinterpret_cast<const cd *>(dev(off_stage64[r][i]));
                 a.tw_lo = reinterpret_cast<const cd *>(dev(off_lo));
                 a.tw_hi = reinterpret_cast<const cd *>(dev(off_hi));
             }

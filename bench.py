@@ -32,8 +32,10 @@ N, BATCH = 4096, 16384
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=10)
+    # the first ~100 back-to-back launches after an idle GPU run up to 15 % slower while its clocks
+    # settle; 200 warm-up launches (~40 ms) measure the sustained rate
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="approximate CPU-baseline budget")
     ap.add_argument("--kernel-only", action="store_true",

@@ -18,11 +18,10 @@ namespace onefft {
 
 // ---- single pass ------------------------------------------------------------------------------
 //
-// Thread j of the transform (0 <= j < 2^LOG2N / P) works on src/dst; `live` false skips the global
-// loads and stores but keeps the thread in every barrier. L is the transform's LDS region of
-// lds_extent<LAYOUT>(2^LOG2N) values (unused when the transform fits one thread). NT: nontemporal
-// global loads and stores. The body is split into its global loads (single_load) and the rest
-// (single_finish), so a pipelined kernel can load the next transform while finishing this one.
+// Thread j of the transform (0 <= j < 2^LOG2N / P) works on the transform that io addresses
+// (fft_common.hpp: PtrIO, BufIO); `live` false skips the global stores but keeps the thread in
+// every barrier. L is the transform's LDS region of lds_extent<LAYOUT>(2^LOG2N) values (unused
+// when the transform fits one thread).
 
 template <int LOG2N>
 struct Single {
@@ -34,29 +33,24 @@ struct Single {
 };
 
 // stage 0's inputs: element j + T q + r N / R_0 into v[q R_0 + r]
-template <int LOG2N, bool INV, bool NT>
-__device__ __forceinline__ void single_load(const cf *__restrict__ src, bool live, uint32_t j,
-                                            cf (&v)[Single<LOG2N>::P]) {
+template <int LOG2N, bool INV, class IO>
+__device__ __forceinline__ void single_load(const IO &io, uint32_t j, cf (&v)[Single<LOG2N>::P]) {
     using G = Single<LOG2N>;
     constexpr int R = 1 << stage_log2r(LOG2N, G::LOG2P, 0);
     constexpr int BF = G::P / R;
     constexpr uint32_t STRIDE = G::N / R;
-    if (live) {
-        static_for<0, BF>([&](auto q) {
-            static_for<0, R>([&](auto r) {
-                constexpr int qq = decltype(q)::value, rr = decltype(r)::value;
-                v[qq * R + rr] = maybe_swap<INV>(gload<NT>(&src[j + G::T * qq + rr * STRIDE]));
-            });
+    static_for<0, BF>([&](auto q) {
+        static_for<0, R>([&](auto r) {
+            constexpr int qq = decltype(q)::value, rr = decltype(r)::value;
+            v[qq * R + rr] = maybe_conj<INV>(io.load(j, G::T * qq + rr * STRIDE));
         });
-    } else {
-        static_for<0, G::P>([&](auto i) { v[i] = mk(0.f, 0.f); });
-    }
+    });
 }
 
-template <int LOG2N, bool INV, bool NT, int LAYOUT, bool TW64 = false>
-__device__ __forceinline__ void single_finish(cf (&v)[Single<LOG2N>::P], cf *__restrict__ dst, bool live,
+template <int LOG2N, bool INV, int LAYOUT, bool TW64, class IO>
+__device__ __forceinline__ void single_finish(cf (&v)[Single<LOG2N>::P], const IO &io, bool live,
                                               const cf *__restrict__ tw, cf *L, uint32_t j,
-                                              const cd *__restrict__ tw64 = nullptr) {
+                                              const cd *__restrict__ tw64) {
     using G = Single<LOG2N>;
     constexpr int LOG2P = G::LOG2P, N = G::N, P = G::P, S = G::S;
     constexpr uint32_t T = G::T;
@@ -77,7 +71,7 @@ __device__ __forceinline__ void single_finish(cf (&v)[Single<LOG2N>::P], cf *__r
             static_for<0, BF>([&](auto q) {
                 static_for<0, R>([&](auto r) {
                     constexpr int qq = decltype(q)::value, rr = decltype(r)::value;
-                    v[qq * R + rr] = L[lds_index<LAYOUT>(j + T * qq + rr * STRIDE)];
+                    v[qq * R + rr] = L[lds_at<LAYOUT>(j + T * qq, rr * STRIDE)];
                 });
             });
         }
@@ -111,7 +105,7 @@ __device__ __forceinline__ void single_finish(cf (&v)[Single<LOG2N>::P], cf *__r
                 static_for<0, BF>([&](auto q) {
                     static_for<0, R>([&](auto m) {
                         constexpr int qq = decltype(q)::value, mm = decltype(m)::value;
-                        gstore<NT>(&dst[j + T * qq + mm * STRIDE], maybe_swap<INV>(v[qq * R + mm]));
+                        io.store(j, T * qq + mm * STRIDE, maybe_conj<INV>(v[qq * R + mm]));
                     });
                 });
             }
@@ -123,7 +117,7 @@ __device__ __forceinline__ void single_finish(cf (&v)[Single<LOG2N>::P], cf *__r
                 const uint32_t base = ((b >> LNS) << (LNS + LR)) + (b & (NS - 1));
                 static_for<0, R>([&](auto m) {
                     constexpr int mm = decltype(m)::value;
-                    L[lds_index<LAYOUT>(base + mm * NS)] = v[qq * R + mm];
+                    L[lds_at<LAYOUT>(base, mm * NS)] = v[qq * R + mm];
                 });
             });
             __syncthreads();
@@ -131,13 +125,13 @@ __device__ __forceinline__ void single_finish(cf (&v)[Single<LOG2N>::P], cf *__r
     });
 }
 
-template <int LOG2N, bool INV, bool NT = false, int LAYOUT = kPadded, bool TW64 = false>
-__device__ __forceinline__ void single_body(const cf *__restrict__ src, cf *__restrict__ dst, bool live,
-                                            const cf *__restrict__ tw, cf *L, uint32_t j,
-                                            const cd *__restrict__ tw64 = nullptr) {
+// `live` false: the loads still run (the caller points io at a valid transform), the stores do not
+template <int LOG2N, bool INV, int LAYOUT, bool TW64, class IO>
+__device__ __forceinline__ void single_body(const IO &io, bool live, const cf *__restrict__ tw, cf *L, uint32_t j,
+                                            const cd *__restrict__ tw64) {
     cf v[Single<LOG2N>::P];
-    single_load<LOG2N, INV, NT>(src, live, j, v);
-    single_finish<LOG2N, INV, NT, LAYOUT, TW64>(v, dst, live, tw, L, j, tw64);
+    single_load<LOG2N, INV>(io, j, v);
+    single_finish<LOG2N, INV, LAYOUT, TW64>(v, io, live, tw, L, j, tw64);
 }
 
 // ---- one tile of one pass of a multi-pass transform -------------------------------------------
@@ -226,7 +220,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, const cf *__restric
             static_for<0, BF>([&](auto q) {
                 static_for<0, R>([&](auto r) {
                     constexpr int qq = decltype(q)::value, rr = decltype(r)::value;
-                    v[qq * R + rr] = L[lds_pad(j + T * qq + rr * STRIDE)];
+                    v[qq * R + rr] = L[lds_at(j + T * qq, rr * STRIDE)];
                 });
             });
         }
@@ -269,7 +263,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, const cf *__restric
                 const uint32_t base = ((b >> LNS) << (LNS + LR)) + (b & (NS - 1));
                 static_for<0, R>([&](auto m) {
                     constexpr int mm = decltype(m)::value;
-                    L[lds_pad(base + mm * NS)] = v[qq * R + mm];
+                    L[lds_at(base, mm * NS)] = v[qq * R + mm];
                 });
             });
             __syncthreads();

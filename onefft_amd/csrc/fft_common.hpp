@@ -46,6 +46,11 @@ ONEFFT_HD cf maybe_swap(cf a) {
     if constexpr (SWAP) return swap_ri(a);
     else return a;
 }
+template <bool CONJ>
+ONEFFT_HD cf maybe_conj(cf a) {
+    if constexpr (CONJ) return mk(a.r, -a.i);
+    else return a;
+}
 
 ONEFFT_HD cd mul64(cd a, cd b) {
     cd c;
@@ -232,6 +237,17 @@ template <int LAYOUT = kPadded>
 __host__ __device__ constexpr uint32_t lds_extent(uint32_t n) {
     return LAYOUT == kSwizzled ? n : n + (n >> 4);
 }
+// lds_index(b + c) with the part that depends on b computed once: for a base b computed at run
+// time and a compile-time step c, the padded index is lds_index(b) + c + (c >> 4) whenever the low
+// four bits of b and c do not carry ((b & 15) + (c & 15) < 16), and the c part folds into the
+// LDS instruction's offset. Every Stockham access here satisfies it: a read takes b < stride and c
+// a multiple of the stride (a power of two); a write takes b = q Ns R + k, k < Ns, and c a multiple
+// of Ns, and Ns R >= 16 or every sum stays below Ns R. The swizzled layout does not fold.
+template <int LAYOUT = kPadded>
+__host__ __device__ constexpr uint32_t lds_at(uint32_t b, uint32_t c) {
+    if constexpr (LAYOUT == kSwizzled) return lds_index<kSwizzled>(b + c);
+    else return lds_index<kPadded>(b) + c + (c >> 4);
+}
 // the padded layout, used by the kernels without a layout choice
 __host__ __device__ constexpr uint32_t lds_pad(uint32_t i) { return lds_index<kPadded>(i); }
 __host__ __device__ constexpr uint32_t lds_span(uint32_t n) { return lds_extent<kPadded>(n); }
@@ -260,5 +276,45 @@ __device__ __forceinline__ void gstore(cf *p, cf c) {
         *p = c;
     }
 }
+
+
+// One transform's global data, addressed as base + (lane + c) complex values with a per-lane
+// part `lane` and a compile-time part `c`.
+//
+// PtrIO: per-lane 64-bit addresses (transforms of different lanes may lie anywhere).
+// BufIO: the transform's base, uniform over the workgroup, in a buffer descriptor held in scalar
+//        registers; the byte offset splits into one per-lane register (lane * 8) and a scalar
+//        offset (c * 8), so an access costs no vector address arithmetic. Offsets are 32-bit:
+//        the transform must lie within 4 GiB of its base, which holds for one transform.
+template <bool NT>
+struct PtrIO {
+    const cf *src;
+    cf *dst;
+    __device__ __forceinline__ cf load(uint32_t lane, uint32_t c) const { return gload<NT>(src + lane + c); }
+    __device__ __forceinline__ void store(uint32_t lane, uint32_t c, cf v) const { gstore<NT>(dst + lane + c, v); }
+};
+
+// word 3 of a gfx9-family buffer descriptor: 32-bit data format, no swizzle, no stride
+constexpr int kBufferWord3 = 0x00020000;
+
+template <bool NT>
+struct BufIO {
+    static constexpr int kAux = NT ? 2 : 0;  // the nt cache-policy bit
+    __amdgpu_buffer_rsrc_t src, dst;
+    __device__ __forceinline__ BufIO(const cf *s, cf *d)
+        : src(__builtin_amdgcn_make_buffer_rsrc(const_cast<cf *>(s), 0, -1, kBufferWord3)),
+          dst(__builtin_amdgcn_make_buffer_rsrc(d, 0, -1, kBufferWord3)) {}
+    // the builtins move the 8 bytes as a GCC-style vector of two 32-bit words
+    typedef unsigned Word __attribute__((vector_size(8)));
+    __device__ __forceinline__ cf load(uint32_t lane, uint32_t c) const {
+        const f2v v = __builtin_bit_cast(f2v, __builtin_amdgcn_raw_buffer_load_b64(src, (int)(lane * 8u),
+                                                                                   (int)(c * 8u), kAux));
+        return mk(v.x, v.y);
+    }
+    __device__ __forceinline__ void store(uint32_t lane, uint32_t c, cf v) const {
+        const f2v w = {v.r, v.i};
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(Word, w), dst, (int)(lane * 8u), (int)(c * 8u), kAux);
+    }
+};
 
 }  // namespace onefft

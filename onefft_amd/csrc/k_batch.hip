@@ -6,7 +6,9 @@
 // src/onefft.c:274-293): every transform is read from HBM once and written once. Each thread
 // holds 16 points in registers; radix-16 (or a smaller first radix) butterflies run in VGPRs and
 // the stages exchange data through padded LDS (Stockham autosort, so the output lands in natural
-// order without a bit-reversal pass).
+// order without a bit-reversal pass). The inverse is conj(F(conj x)), the same transform as the
+// reference's re/im exchange around F (src/onefft.c:86-88) up to rounding: the negations fold
+// into the first and last butterflies' additions, where the exchange cost register moves.
 #include "fft_bodies.hpp"
 
 namespace onefft {
@@ -25,10 +27,21 @@ __global__ __launch_bounds__(batch_wg(LOG2N)) void fft_batch_kernel(const cf *__
     __shared__ cf lds[USES_LDS ? PER * SPAN : 1];
 
     const uint32_t t = threadIdx.x;
-    const uint32_t c = t / T;
-    const uint64_t which = (uint64_t)blockIdx.x * PER + c;
-    single_body<LOG2N, INV, NT, LAYOUT, TW64>(in + which * in_dist, out + which * out_dist, which < batch, tw,
-                                              lds + (USES_LDS ? c * SPAN : 0), t % T, tw64);
+    if constexpr (PER == 1) {
+        // one transform per workgroup, and the grid has exactly one workgroup per transform: its
+        // base is uniform, so it goes into buffer descriptors
+        const uint64_t which = blockIdx.x;
+        const BufIO<NT> io(in + which * in_dist, out + which * out_dist);
+        single_body<LOG2N, INV, LAYOUT, TW64>(io, true, tw, lds, t, tw64);
+    } else {
+        // the last workgroup's lanes past the batch load the batch's last transform again and
+        // store nothing
+        const uint32_t c = t / T;
+        const uint64_t which = (uint64_t)blockIdx.x * PER + c;
+        const uint64_t src = which < batch ? which : batch - 1;
+        const PtrIO<NT> io{in + src * in_dist, out + which * out_dist};
+        single_body<LOG2N, INV, LAYOUT, TW64>(io, which < batch, tw, lds + (USES_LDS ? c * SPAN : 0), t % T, tw64);
+    }
 }
 
 template <int LOG2N, bool NT, int LAYOUT, bool TW64 = false>

@@ -252,6 +252,19 @@ __host__ __device__ constexpr uint32_t lds_at(uint32_t b, uint32_t c) {
 __host__ __device__ constexpr uint32_t lds_pad(uint32_t i) { return lds_index<kPadded>(i); }
 __host__ __device__ constexpr uint32_t lds_span(uint32_t n) { return lds_extent<kPadded>(n); }
 
+// ---- workgroup order --------------------------------------------------------------------------
+//
+// The GPU hands the workgroups of a launch to its 8 XCDs in turn (workgroup i to XCD i mod 8), and
+// each XCD has its own L2. Renumbering workgroup b of `count` so that XCD x runs one contiguous
+// range of the work makes neighbouring tiles, which share cache lines, DRAM pages and page-table
+// entries, meet in the same L2. A bijection on [0, count) for any count; on a device with another
+// number of XCDs it only changes the order.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_contiguous(uint32_t b, uint32_t count) {
+    const uint32_t q = count / kXcds, r = count % kXcds, x = b % kXcds, i = b / kXcds;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
 // ---- global access --------------------------------------------------------------------------
 //
 // Streamed data (read once, written once) can use nontemporal loads and stores, which keep it

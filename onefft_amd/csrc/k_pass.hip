@@ -25,9 +25,12 @@ template <int LOG2NF, int C, bool FIRST, bool SWAP_IN, bool SWAP_OUT>
 __global__ __launch_bounds__(pass_wg(LOG2NF, C)) void fft_pass_kernel(PassArgs a) {
     constexpr bool USES_LDS = stage_count(LOG2NF, pass_log2p(LOG2NF)) > 1;
     __shared__ cf lds[USES_LDS ? C * pass_span(LOG2NF) : 1];
-    pass_body<LOG2NF, C, FIRST, SWAP_IN, SWAP_OUT>(a, a.in + (uint64_t)blockIdx.y * a.in_dist,
-                                                   a.out + (uint64_t)blockIdx.y * a.out_dist, blockIdx.x, lds,
-                                                   threadIdx.x);
+    // tiles in XCD-contiguous order over the whole grid (tile-major within a transform)
+    const uint32_t tiles = gridDim.x;
+    const uint32_t w = xcd_contiguous(blockIdx.x + blockIdx.y * tiles, tiles * gridDim.y);
+    const uint32_t tile = w % tiles, which = w / tiles;
+    pass_body<LOG2NF, C, FIRST, SWAP_IN, SWAP_OUT>(a, a.in + (uint64_t)which * a.in_dist,
+                                                   a.out + (uint64_t)which * a.out_dist, tile, lds, threadIdx.x);
 }
 
 template <int LOG2NF, int C>

@@ -7,9 +7,10 @@ the tests and the benchmark use:
 * ``Plan`` — owns an ``onefft_plan`` (reference ``include/onefft.h:88-109``);
 * ``fft(x, inverse=False)`` — transforms a complex64 CUDA tensor along its last dimension through
   the batched device entry point on torch's current stream;
-* ``FFT`` — a ``torch.autograd.Function`` whose backward is the adjoint: the adjoint of the
-  unnormalised forward DFT is the unnormalised inverse (and vice versa), run by the same kernels
-  and tables.
+* ``fft_autograd(x, inverse=False)`` — the same transform as the torch custom op ``onefft::fft``
+  (``onefft_amd/torch_op.py``): differentiable, with fake-tensor support for torch.compile; its
+  backward is the adjoint: the adjoint of the unnormalised forward DFT is the unnormalised
+  inverse (and vice versa), run by the same kernels and tables.
 
 There is no fallback: if the library is missing, every call raises.
 """
@@ -237,14 +238,23 @@ class Graph:
             pass
 
 
-_plans: dict[int, Plan] = {}
+_plans: dict[tuple, Plan] = {}
 
 
-def plan(n: int) -> Plan:
-    """A process-wide plan per size (created on first use, like the library's own cache)."""
-    p = _plans.get(n)
+def plan(n: int, device: int | None = None) -> Plan:
+    """A process-wide plan per (device, size), created on first use like the library's own cache.
+    `device` None means HIP's current device; a plan lives on the device current at its creation."""
+    key = (device, n)
+    p = _plans.get(key)
     if p is None:
-        p = _plans[n] = Plan(n)
+        if device is None:
+            p = Plan(n)
+        else:
+            import torch
+
+            with torch.cuda.device(device):
+                p = Plan(n)
+        p = _plans.setdefault(key, p)
     return p
 
 
@@ -254,51 +264,30 @@ def _stream_ptr(torch, device) -> int:
 
 def fft(x, inverse: bool = False, out=None):
     """Unnormalised DFT of a complex64 CUDA tensor along its last dimension (power-of-two length),
-    asynchronous on torch's current stream."""
+    asynchronous on torch's current stream. The result is a new contiguous tensor (or `out`, which
+    must be a contiguous complex64 tensor of x's shape on x's device)."""
     import torch
 
     if x.dtype != torch.complex64 or not x.is_cuda:
         raise OneFFTError("fft expects a complex64 CUDA tensor")
-    x = x.contiguous()
-    n = x.shape[-1]
+    # materialise lazy conjugation / negation: the kernels read the tensor's memory as stored
+    x = x.resolve_conj().resolve_neg().contiguous()
+    n = x.shape[-1] if x.dim() else 1
     batch = x.numel() // n if n else 0
     if out is None:
         out = torch.empty_like(x)
+    elif (out.shape != x.shape or out.dtype != x.dtype or out.device != x.device or not out.is_contiguous()
+          or out.is_conj() or out.is_neg()):
+        raise OneFFTError("out must be a contiguous complex64 tensor of the input's shape and device")
     if batch == 0:
         return out
-    plan(n).batch_ptr(x.data_ptr(), out.data_ptr(), batch, n, n, inverse, _stream_ptr(torch, x.device))
+    plan(n, x.device.index).batch_ptr(x.data_ptr(), out.data_ptr(), batch, n, n, inverse,
+                                      _stream_ptr(torch, x.device))
     return out
 
 
-def _autograd():
-    import torch
-
-    class FFT(torch.autograd.Function):
-        """y = F x (forward) or F^H x (inverse), unnormalised, along the last dimension.
-
-        Backward applies the adjoint: for y = F x, dL/dx = F^H dL/dy (PyTorch's conjugate
-        Wirtinger convention), and F^H is the unnormalised inverse transform, so the backward of
-        one direction is the other direction on the same plan.
-        """
-
-        @staticmethod
-        def forward(ctx, x, inverse: bool = False):
-            ctx.inverse = bool(inverse)
-            return fft(x, inverse=bool(inverse))
-
-        @staticmethod
-        def backward(ctx, g):
-            return fft(g.contiguous(), inverse=not ctx.inverse), None
-
-    return FFT
-
-
-_FFT = None
-
-
 def fft_autograd(x, inverse: bool = False):
-    """Differentiable unnormalised FFT (see FFT)."""
-    global _FFT
-    if _FFT is None:
-        _FFT = _autograd()
-    return _FFT.apply(x, inverse)
+    """Differentiable unnormalised FFT along the last dimension: the torch custom op onefft::fft."""
+    from . import torch_op
+
+    return torch_op.fft_op(x, bool(inverse))

@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the timed transform launches plus one 1 GiB device copy (for counter runs)")
     ap.add_argument("--no-extra", action="store_true", help="skip the other workloads")
+    ap.add_argument("--report", action="store_true",
+                    help="add a per-configuration report: errors against the reference CPU transform and the "
+                         "FP64 oracle, and the reference's CPU times (several minutes of host work)")
     ap.add_argument("--variants", type=str, default=None,
                     help="instead of the benchmark: interleaved A/B of the single-pass kernel variants at these "
                          "log2 sizes (comma list), 512 MiB in and out each")
@@ -210,6 +213,187 @@ def extra_workloads(torch, F, stream):
     return out
 
 
+def report(torch, F, stream, x2, y2, ms2, threads):
+    """--report: per configuration, the GPU time, the error against the reference CPU transform
+    and the FP64 oracle, and the reference's own CPU time on `threads` host cores (single-thread
+    where the reference cannot thread a transform). x2, y2: the headline batch and its output,
+    ms2 its time per launch. Several minutes of host work (FP64 oracle of 2^27 points)."""
+    import numpy as np
+    from oracle import oracle as O
+
+    sp = stream.cuda_stream
+    T = threads
+    have_ref = O.ref() is not None
+    out = {"cpu_threads": T, "cpu_isa": O.ref_implementation() if have_ref else None}
+
+    def progress(what):
+        print(f"report: {what}", file=sys.stderr, flush=True)
+
+    def dev(a):  # interleaved float32 -> complex64 CUDA tensor
+        return torch.from_numpy(a.view(np.complex64)).cuda()
+
+    def host(t):  # complex64 tensor -> interleaved float32
+        return t.cpu().numpy().view(np.float32).reshape(-1)
+
+    def cpu_time(x, n, batch, thr, inverse=False, samples=5, seconds=0.3):
+        return O.ref_time_batch(x, n, batch, thr, inverse, sample_seconds=seconds, samples=samples) if have_ref \
+            else float("nan")
+
+    progress("config 1")
+    # 1. n = 1024 forward + inverse, U[-1,1) seed 0, host pointers (the drop-in path)
+    n = 1024
+    x = O.uniform(0, n)
+    y, z = np.empty_like(x), np.empty_like(x)
+    p = F.plan(n)
+    for _ in range(20):
+        p.forward_ptr(x.ctypes.data, y.ctypes.data)
+    t0 = time.perf_counter()
+    for _ in range(2000):
+        p.forward_ptr(x.ctypes.data, y.ctypes.data)
+        p.inverse_ptr(y.ctypes.data, z.ctypes.data)
+    host_us = (time.perf_counter() - t0) / 2000 * 1e6
+    xd, yd = dev(x), torch.empty(n, dtype=torch.complex64, device="cuda")
+    dev_us = time_ms(torch, stream, lambda: p.batch_ptr(xd.data_ptr(), yd.data_ptr(), 1, n, n, False, sp), 2000) * 1e3
+    c1 = cpu_time(x, n, 1, 1) + cpu_time(y, n, 1, 1, True)
+    xt = np.tile(x, 256 * T)
+    cT = cpu_time(xt, n, 256 * T, T)
+    out["1_n1024"] = {
+        "host_fwd_inv_us": round(host_us, 2), "device_fwd_us": round(dev_us, 2),
+        "rel_l2_vs_ref": O.rel_rms(y, O.ref_fft(x, n)) if have_ref else None,
+        "rel_l2_vs_fp64": O.rel_rms(y, O.fft_f64(x, n)), "roundtrip_rel_l2": O.rel_rms(z, n * x),
+        "cpu_fwd_inv_us_1thread": round(c1 * 1e6, 3),
+        "cpu_fwd_gflops_T": round(5 * n * 10 * 256 * T / cT / 1e9, 1)}
+
+    progress("config 2")
+    # 2. the headline batch: errors on a sample of its transforms
+    n, sample = N, 256
+    xs = host(x2[:sample])
+    ys = host(y2[:sample])
+    out["2_batch4096"] = {
+        "us": round(ms2 * 1e3, 2), "gflops": round(5 * n * 12 * BATCH / (ms2 * 1e-3) / 1e9, 1),
+        "rel_l2_vs_ref_256": O.rel_rms(ys, O.ref_fft(xs, n, batch=sample, threads=T)) if have_ref else None,
+        "rel_l2_vs_fp64_32": O.rel_rms(ys[:2 * n * 32], O.fft_f64(xs[:2 * n * 32], n, batch=32))}
+
+    progress("config 3")
+    # 3. large single transforms: U[-1,1) seed 2 plus a unit tone at bin round(n / 7)
+    rows = []
+    for L in (22, 24, 27):
+        n = 1 << L
+        x = O.add_tone(O.uniform(2, n), n, int(round(n / 7)))
+        xd = dev(x)
+        yd, zd = torch.empty_like(xd), torch.empty_like(xd)
+        pl = F.Plan(n)
+        fwd = lambda: pl.batch_ptr(xd.data_ptr(), yd.data_ptr(), 1, n, n, False, sp)
+        inv = lambda: pl.batch_ptr(yd.data_ptr(), zd.data_ptr(), 1, n, n, True, sp)
+        fwd(), inv()
+        torch.cuda.synchronize()
+        reps = 20 if L < 27 else 5
+        tf, ti = time_ms(torch, stream, fwd, reps), time_ms(torch, stream, inv, reps)
+        y, z = host(yd), host(zd)
+        samples = 5 if L < 27 else 2
+        row = {"log2n": L, "fwd_ms": round(tf, 4), "inv_ms": round(ti, 4),
+               "fwd_gflops": round(5 * n * L / (tf * 1e-3) / 1e9, 1),
+               "rel_l2_vs_fp64": O.rel_rms(y, O.fft_f64(x, n)),
+               "rel_l2_vs_ref": O.rel_rms(y, O.ref_fft(x, n)) if have_ref else None,
+               "roundtrip_rel_l2": O.rel_rms(z, n * x),
+               "cpu_fwd_ms_1thread": round(cpu_time(x, n, 1, 1, samples=samples, seconds=0.5) * 1e3, 2)}
+        row["cpu_fwd_gflops_1thread"] = round(5 * n * L / (row["cpu_fwd_ms_1thread"] * 1e-3) / 1e9, 2)
+        rows.append(row)
+        progress(f"config 3, 2^{L} done")
+        pl.close()
+        del xd, yd, zd
+        torch.cuda.empty_cache()
+    out["3_large"] = rows
+
+    progress("config 4")
+    # 4. ragged: the 4 GiB prefix of the seed-3 stream, N(0,1) inputs from seed 3
+    items = F.ragged_items(3, 200000, max_bytes=4 << 30)
+    extent = items[-1][0] + (1 << items[-1][1])
+    x = O.normal(3, extent)
+    xd = dev(x)
+    yd = torch.zeros_like(xd)
+    rp = F.RaggedPlan(items)
+    rp.execute(xd.data_ptr(), yd.data_ptr(), sp)
+    rp.wait(sp)
+    t_one = time_ms(torch, stream, lambda: rp.execute(xd.data_ptr(), yd.data_ptr(), sp), 5)
+    rp.wait(sp)
+    y = host(yd)
+    worst_ref = worst_bar = worst64 = 0.0
+    if have_ref:
+        yr, secs = O.ref_ragged(x, items, threads=T, reps=3)
+    for i, (off, l, inv) in enumerate(items):
+        seg = slice(2 * off, 2 * (off + (1 << l)))
+        if have_ref:
+            e = O.rel_rms(y[seg], yr[seg])
+            worst_ref = max(worst_ref, e)
+            worst_bar = max(worst_bar, e / (3e-7 * l))
+        if l <= 16 or i < 400:  # the FP64 oracle on every small item and on the first items of each size
+            worst64 = max(worst64, O.rel_rms(y[seg], O.fft_f64(x[seg], 1 << l, bool(inv))))
+    flops = sum(5.0 * (1 << l) * l for _, l, _ in items)
+    rp.close()
+    out["4_ragged"] = {
+        "items": len(items), "one_launch_ms": round(t_one, 3), "gflops": round(flops / (t_one * 1e-3) / 1e9, 1),
+        "max_rel_l2_vs_ref": worst_ref if have_ref else None,
+        "max_rel_l2_vs_ref_over_bar": worst_bar if have_ref else None,
+        "max_rel_l2_vs_fp64_checked_items": worst64,
+        "cpu_ms_T": round(secs * 1e3, 2) if have_ref else None,
+        "cpu_gflops_T": round(flops / secs / 1e9, 1) if have_ref else None}
+
+    progress("config 4b")
+    # 4b. every one of the 200000 items, in chunks of <= 4 GiB input, one launch per chunk
+    del xd, yd
+    torch.cuda.empty_cache()
+    allitems = F.ragged_items(3, 200000)
+    chunks, start = [], 0
+    while start < len(allitems):
+        base, end = allitems[start][0], start
+        while end < len(allitems) and (allitems[end][0] - base + (1 << allitems[end][1])) * 8 <= (4 << 30):
+            end += 1
+        chunks.append([(o - base, l, v) for o, l, v in allitems[start:end]])
+        start = end
+    extents = [ch[-1][0] + (1 << ch[-1][1]) for ch in chunks]
+    # buffers for the longest chunk: every chunk's items lie inside them
+    xd = torch.empty(max(extents), dtype=torch.complex64, device="cuda")
+    yd = torch.empty_like(xd)
+    total_ms, total_flops, total_bytes = 0.0, 0.0, 0.0
+    for ch, ext in zip(chunks, extents):
+        assert ext <= xd.numel() and ext <= yd.numel()
+        xd[:ext].normal_()
+        rp = F.RaggedPlan(ch)
+        assert rp.extent == ext
+        rp.execute(xd.data_ptr(), yd.data_ptr(), sp)
+        rp.wait(sp)
+        total_ms += time_ms(torch, stream, lambda: rp.execute(xd.data_ptr(), yd.data_ptr(), sp), 1)
+        rp.wait(sp)
+        rp.close()
+        total_flops += sum(5.0 * (1 << l) * l for _, l, _ in ch)
+        total_bytes += 16.0 * ext
+    out["4b_ragged_all_items"] = {"items": len(allitems), "chunks": len(chunks), "gpu_ms": round(total_ms, 2),
+                                  "gflops": round(total_flops / (total_ms * 1e-3) / 1e9, 1),
+                                  "alg_gb_s": round(total_bytes / (total_ms * 1e-3) / 1e9, 1),
+                                  "gib_in": round(total_bytes / 2 / 2**30, 1)}
+    del xd, yd
+    torch.cuda.empty_cache()
+
+    progress("config 5")
+    # 5. spectral filter: the FFT part, 256 x 65536 forward and inverse, N(0,1) seed 4
+    B, n = 256, 65536
+    x = O.normal(4, B * n)
+    xd = dev(x)
+    yd, zd = torch.empty_like(xd), torch.empty_like(xd)
+    pf = F.plan(n)
+    tf = time_ms(torch, stream, lambda: pf.batch_ptr(xd.data_ptr(), yd.data_ptr(), B, n, n, False, sp), 10)
+    ti = time_ms(torch, stream, lambda: pf.batch_ptr(yd.data_ptr(), zd.data_ptr(), B, n, n, True, sp), 10)
+    y = host(yd)
+    cf = cpu_time(x, n, B, T) + cpu_time(y, n, B, T, True)
+    out["5_filter_fft"] = {
+        "fwd_inv_ms": round(tf + ti, 4), "gflops": round(2 * 5 * n * 16 * B / ((tf + ti) * 1e-3) / 1e9, 1),
+        "rel_l2_vs_ref_16": O.rel_rms(y[:2 * n * 16], O.ref_fft(x[:2 * n * 16], n, batch=16)) if have_ref else None,
+        "rel_l2_vs_fp64_4": O.rel_rms(y[:2 * n * 4], O.fft_f64(x[:2 * n * 4], n, batch=4)),
+        "cpu_fwd_inv_ms_T": round(cf * 1e3, 2), "cpu_gflops_T": round(2 * 5 * n * 16 * B / cf / 1e9, 1)}
+    return out
+
+
 def traffic_from_profiles(name: str):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by a
     copy-kernel calibration), committed under profiles/; None when not measured."""
@@ -330,6 +514,11 @@ def main():
 
     extras = {} if args.no_extra else extra_workloads(torch, F, stream)
     cpu = {"value": None, "skipped": True} if args.no_cpu else cpu_baseline(args.cpu_seconds)
+    if args.report:
+        from oracle import oracle as O
+
+        threads = max(1, min(16, O.ref_online_cpus() or (os.cpu_count() or 1)))
+        extras["report"] = report(torch, F, stream, x, y, ms, threads)
 
     line = {
         "metric": "5n·log2n GFLOP/s & %HBM peak: batched vs per-call, rel-L2 <= 3e-7·log2 n",

@@ -68,10 +68,12 @@ def test_gradients_against_fp64_finite_differences():
         eps = 1e-2
         fd = (loss64(x + eps * v, H, t64) - loss64(x - eps * v, H, t64)) / (2 * eps)
         ad = float(np.sum(np.conj(gx.astype(np.complex128)) * v).real)
+        print(f"d/dx along v{trial}: relative difference to FP64 central difference {abs(ad - fd) / abs(fd):.3g}")
         assert abs(ad - fd) <= 1e-5 * abs(fd), (trial, ad, fd)
         w = (rng.standard_normal(N) + 1j * rng.standard_normal(N)).astype(np.complex128)
         fd = (loss64(x, H + eps * w, t64) - loss64(x, H - eps * w, t64)) / (2 * eps)
         ad = float(np.sum(np.conj(gH.astype(np.complex128)) * w).real)
+        print(f"d/dH along w{trial}: relative difference to FP64 central difference {abs(ad - fd) / abs(fd):.3g}")
         assert abs(ad - fd) <= 1e-5 * abs(fd), (trial, ad, fd)
 
 
@@ -93,6 +95,7 @@ def test_gradients_against_reference_transform():
     gH_ref = np.sum(np.conj(X) * g_u, axis=0)
     for got, want in ((gx, gx_ref), (gH, gH_ref)):
         err = np.linalg.norm(got - want) / np.linalg.norm(want)
+        print(f"gradient rel-L2 against the reference-built gradient: {err:.3g}")
         assert err < 3e-7 * 16 * 4, err  # four FP32 transforms of 2^16 points on each side
 
 

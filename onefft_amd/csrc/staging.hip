@@ -2,29 +2,61 @@
 // or device pointers of any alignment (the reference accepts any alignment,
 // include/onefft.h:61-64; its tests misalign by 4 and 12 bytes, tests/test_onefft.c:118-119).
 //
-// Each calling thread owns a stream and a pair of device staging buffers, grown to the largest
-// size it has transformed and kept until the thread exits. Host data (and device data that is not
-// 8-byte aligned) is copied into the staging buffers, transformed there, and copied back; device
-// data that is aligned is transformed where it lies. Plans stay read-only, so threads never
-// contend (reference include/onefft.h:102-104).
+// Each calling thread owns a stream, a pair of device staging buffers (grown to the largest size
+// it has transformed) and a ring of pinned host chunks, all kept until the thread exits. Plans
+// stay read-only, so threads never contend (reference include/onefft.h:102-104). Per argument:
+//   * device memory, 8-byte aligned: transformed where it lies;
+//   * device memory, misaligned: copied device to device into the staging buffer and back;
+//   * pinned host memory: copied by DMA straight to and from the staging buffers;
+//   * ordinary (pageable) host memory, transforms of one LDS pass (n <= 2^14, 128 KiB): copied by
+//     the CPU into a per-thread coherent pinned buffer that the kernel reads and writes directly
+//     over the host link (no DMA transfers, whose set-up latency dominates at these sizes);
+//   * ordinary (pageable) host memory up to 4 MiB: HIP's own pageable copies (lower set-up latency
+//     than the ring below at these sizes);
+//   * ordinary (pageable) host memory, larger: a pipeline through the pinned ring. The input is cut into
+//     chunks; the CPU copies chunk i into a free pinned slot while the DMA engine moves chunk
+//     i - 1 to the device, and the output comes back the same way in reverse, so the CPU copy
+//     and the link run at the same time. (HIP's own pageable copies run at ~30 GB/s host to
+//     device and ~18 GB/s back on MI355X hosts; pinned DMA at ~57 GB/s each way.)
+// The output is written only after the transform has completed without error.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include "../../include/onefft.h"
+#include "kernels.hpp"
 #include "runtime.hpp"
 
 namespace onefft {
 namespace {
+
+constexpr size_t kChunk = size_t(4) << 20;  // bytes per pinned slot
+constexpr int kSlots = 4;                   // slots in the ring: 16 MiB pinned per thread
+constexpr size_t kDirectBytes = size_t(8) << kMaxSinglePassLog2;  // one LDS-pass transform
+constexpr size_t kRingMinBytes = kChunk;                          // smaller: HIP's pageable copies
 
 struct Staging {
     int device = -1;
     hipStream_t stream = nullptr;
     void *buf = nullptr;  // 2 * cap bytes: input then output
     size_t cap = 0;
+    char *pinned = nullptr;  // kSlots * kChunk bytes of pinned host memory
+    hipEvent_t ev[kSlots] = {};
+    char *direct = nullptr;      // 2 * kDirectBytes of coherent pinned host memory: input, output
+    char *direct_dev = nullptr;  // the same memory as the device addresses it
 
     void release() {
+        if (stream) (void)hipStreamSynchronize(stream);
         if (buf) (void)hipFree(buf);
+        if (pinned) (void)hipHostFree(pinned);
+        if (direct) (void)hipHostFree(direct);
+        direct = direct_dev = nullptr;
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto &e : ev) e = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
         buf = nullptr;
+        pinned = nullptr;
         stream = nullptr;
         cap = 0;
         device = -1;
@@ -34,13 +66,127 @@ struct Staging {
 
 thread_local Staging t_stage;
 
-bool is_device_pointer(const void *p) {
+enum class Mem { kPageable, kPinned, kDevice };
+
+Mem memory_kind(const void *p) {
     hipPointerAttribute_t attr;
     if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
         (void)hipGetLastError();  // plain host memory is not an error for us
+        return Mem::kPageable;
+    }
+    if (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged) return Mem::kDevice;
+    if (attr.type == hipMemoryTypeHost) return Mem::kPinned;
+    return Mem::kPageable;
+}
+
+// the pinned ring and its events; all or nothing (s.pinned is set last)
+bool ring_reserve(Staging &s) {
+    if (s.pinned) return true;
+    for (auto &e : s.ev) {
+        if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            e = nullptr;
+            set_error("staging event creation failed");
+            return false;
+        }
+    }
+    char *p = nullptr;
+    if (hipHostMalloc(reinterpret_cast<void **>(&p), kSlots * kChunk, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("pinned staging allocation failed");
         return false;
     }
-    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+    s.pinned = p;
+    return true;
+}
+
+// the coherent pinned buffer pair the single-pass kernels read and write in place
+bool direct_reserve(Staging &s) {
+    if (s.direct) return true;
+    char *p = nullptr;
+    void *d = nullptr;
+    if (hipHostMalloc(reinterpret_cast<void **>(&p), 2 * kDirectBytes,
+                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("pinned staging allocation failed");
+        return false;
+    }
+    if (hipHostGetDevicePointer(&d, p, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipHostFree(p);
+        set_error("pinned staging mapping failed");
+        return false;
+    }
+    s.direct = p;
+    s.direct_dev = static_cast<char *>(d);
+    return true;
+}
+
+// pageable host -> device, through the pinned ring; asynchronous on s.stream when it returns
+bool stage_in(Staging &s, void *d_dst, const void *src, size_t bytes) {
+    const size_t chunks = (bytes + kChunk - 1) / kChunk;
+    for (size_t i = 0; i < chunks; i++) {
+        const int slot = int(i % kSlots);
+        const size_t off = i * kChunk, len = bytes - off < kChunk ? bytes - off : kChunk;
+        if (i >= size_t(kSlots)) {
+            const hipError_t e = hipEventSynchronize(s.ev[slot]);
+            if (e != hipSuccess) {
+                set_error("input copy", e);
+                return false;
+            }
+        }
+        char *pin = s.pinned + slot * kChunk;
+        std::memcpy(pin, static_cast<const char *>(src) + off, len);
+        hipError_t e = hipMemcpyAsync(static_cast<char *>(d_dst) + off, pin, len, hipMemcpyHostToDevice, s.stream);
+        if (e == hipSuccess) e = hipEventRecord(s.ev[slot], s.stream);
+        if (e != hipSuccess) {
+            set_error("input copy", e);
+            return false;
+        }
+    }
+    return true;
+}
+
+// device -> pageable host, through the pinned ring, after everything queued on s.stream;
+// synchronous. Nothing is written to dst unless the queued work completed without error.
+bool stage_out(Staging &s, void *dst, const void *d_src, size_t bytes) {
+    const size_t chunks = (bytes + kChunk - 1) / kChunk;
+    auto enqueue = [&](size_t i) {
+        const int slot = int(i % kSlots);
+        const size_t off = i * kChunk, len = bytes - off < kChunk ? bytes - off : kChunk;
+        hipError_t e = hipMemcpyAsync(s.pinned + slot * kChunk, static_cast<const char *>(d_src) + off, len,
+                                      hipMemcpyDeviceToHost, s.stream);
+        if (e == hipSuccess) e = hipEventRecord(s.ev[slot], s.stream);
+        return e;
+    };
+    for (size_t i = 0; i < chunks && i < size_t(kSlots); i++) {
+        const hipError_t e = enqueue(i);
+        if (e != hipSuccess) {
+            set_error("output copy", e);
+            (void)hipStreamSynchronize(s.stream);
+            return false;
+        }
+    }
+    for (size_t i = 0; i < chunks; i++) {
+        const int slot = int(i % kSlots);
+        const size_t off = i * kChunk, len = bytes - off < kChunk ? bytes - off : kChunk;
+        const hipError_t e = hipEventSynchronize(s.ev[slot]);
+        if (e != hipSuccess) {
+            set_error(i == 0 ? "transform" : "output copy", e);
+            (void)hipStreamSynchronize(s.stream);
+            return false;
+        }
+        std::memcpy(static_cast<char *>(dst) + off, s.pinned + slot * kChunk, len);
+        if (i + kSlots < chunks) {
+            const hipError_t e2 = enqueue(i + kSlots);
+            if (e2 != hipSuccess) {
+                set_error("output copy", e2);
+                (void)hipStreamSynchronize(s.stream);
+                return false;
+            }
+        }
+    }
+    return true;
 }
 
 }  // namespace
@@ -62,6 +208,8 @@ hipStream_t thread_stream(int device) {
 bool staging_reserve(int device, size_t n) {
     if (!thread_stream(device)) return false;
     Staging &s = t_stage;
+    if (n && !ring_reserve(s)) return false;
+    if (n && n * 2 * sizeof(float) <= kDirectBytes && !direct_reserve(s)) return false;
     const size_t bytes = n * 2 * sizeof(float);
     if (s.cap >= bytes) return true;
     if (s.buf) {
@@ -88,19 +236,43 @@ int execute_host_call(const onefft_plan *plan, const float *in, float *out, bool
         set_error("no HIP device");
         return -1;
     }
-    const bool in_direct = is_device_pointer(in) && ((uintptr_t)in & 7) == 0;
-    const bool out_direct = is_device_pointer(out) && ((uintptr_t)out & 7) == 0;
+    const Mem in_kind = memory_kind(in), out_kind = memory_kind(out);
+    const bool in_direct = in_kind == Mem::kDevice && ((uintptr_t)in & 7) == 0;
+    const bool out_direct = out_kind == Mem::kDevice && ((uintptr_t)out & 7) == 0;
     if (!staging_reserve(device, (in_direct && out_direct) ? 0 : n)) return -1;
     Staging &s = t_stage;
+    if (in_kind == Mem::kPageable && out_kind == Mem::kPageable && bytes <= kDirectBytes) {
+        std::memcpy(s.direct, in, bytes);
+        const cf *h_in = reinterpret_cast<const cf *>(s.direct_dev);
+        cf *h_out = reinterpret_cast<cf *>(s.direct_dev + kDirectBytes);
+        if (execute_device(plan, h_in, h_out, 1, n, n, inverse, s.stream) != 0) {
+            (void)hipStreamSynchronize(s.stream);
+            return -1;
+        }
+        const hipError_t e = hipStreamSynchronize(s.stream);
+        if (e != hipSuccess) {
+            set_error("transform", e);
+            return -1;
+        }
+        std::memcpy(out, s.direct + kDirectBytes, bytes);
+        return 0;
+    }
     const cf *d_in = reinterpret_cast<const cf *>(in);
     cf *d_out = reinterpret_cast<cf *>(out);
-    hipError_t e;
     if (!in_direct) {
         d_in = reinterpret_cast<const cf *>(s.buf);
-        e = hipMemcpyAsync(s.buf, in, bytes, hipMemcpyDefault, s.stream);
-        if (e != hipSuccess) {
-            set_error("input copy", e);
-            return -1;
+        if (in_kind == Mem::kPageable && bytes >= kRingMinBytes) {
+            if (!stage_in(s, s.buf, in, bytes)) {
+                (void)hipStreamSynchronize(s.stream);
+                return -1;
+            }
+        } else {
+            const hipError_t e = hipMemcpyAsync(s.buf, in, bytes, hipMemcpyDefault, s.stream);
+            if (e != hipSuccess) {
+                set_error("input copy", e);
+                (void)hipStreamSynchronize(s.stream);
+                return -1;
+            }
         }
     }
     if (!out_direct) d_out = reinterpret_cast<cf *>(static_cast<char *>(s.buf) + s.cap);
@@ -108,7 +280,17 @@ int execute_host_call(const onefft_plan *plan, const float *in, float *out, bool
         (void)hipStreamSynchronize(s.stream);
         return -1;
     }
+    if (!out_direct && out_kind == Mem::kPageable && bytes >= kRingMinBytes)
+        return stage_out(s, out, d_out, bytes) ? 0 : -1;
+    hipError_t e = hipSuccess;
     if (!out_direct) {
+        // pinned or small pageable host output, or misaligned device output: one copy after the
+        // transform has completed
+        e = hipStreamSynchronize(s.stream);
+        if (e != hipSuccess) {
+            set_error("transform", e);
+            return -1;
+        }
         e = hipMemcpyAsync(out, d_out, bytes, hipMemcpyDefault, s.stream);
         if (e != hipSuccess) {
             set_error("output copy", e);

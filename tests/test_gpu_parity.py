@@ -84,6 +84,38 @@ def check(got, x, n, inverse, label):
 
 # ---- the drop-in path: host pointers, misaligned, guard bands -------------------------------------
 
+@pytest.mark.parametrize("L", [22, 23])
+def test_host_pipeline_bitwise_equals_device_path(L):
+    """Host buffers larger than the pinned staging ring (4 x 4 MiB, so its slots are reused):
+    pageable (misaligned, guarded) and pinned host memory give the device path's bits."""
+    n = 1 << L
+    print(f"host pipeline n=2^{L}", flush=True)
+    p = F.plan(n)
+    x = O.uniform(40 + L, n)
+    d_in = torch.from_numpy(x).cuda()
+    for inverse in (False, True):
+        d_out = torch.empty_like(d_in)
+        p.batch_ptr(d_in.data_ptr(), d_out.data_ptr(), 1, n, n, inverse, stream())
+        torch.cuda.synchronize()
+        want = d_out.cpu().numpy()
+        call = p.inverse_ptr if inverse else p.forward_ptr
+        src, dst = Guarded(2 * n, 1), Guarded(2 * n, 3)
+        src.data[:] = x
+        call(src.ptr, dst.ptr)
+        assert np.array_equal(dst.data, want) and dst.intact() and src.intact()
+        pin_in = torch.from_numpy(x).pin_memory()
+        pin_out = torch.zeros(2 * n, dtype=torch.float32).pin_memory()
+        call(pin_in.data_ptr(), pin_out.data_ptr())
+        assert np.array_equal(pin_out.numpy(), want)
+        # pinned input, pageable output and the reverse
+        dst.data[:] = 0
+        call(pin_in.data_ptr(), dst.ptr)
+        assert np.array_equal(dst.data, want) and dst.intact()
+        pin_out.zero_()
+        call(src.ptr, pin_out.data_ptr())
+        assert np.array_equal(pin_out.numpy(), want)
+
+
 @pytest.mark.parametrize("L", list(range(0, 21)))
 def test_plan_host_pointers(L):
     n = 1 << L

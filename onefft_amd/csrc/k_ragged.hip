@@ -3,15 +3,21 @@
 //
 // The reference transforms one array per call (include/onefft.h:61-77); a caller with many
 // arrays of mixed sizes calls it once per array. Here every array of a batch is an item
-// (offset, log2 n, direction). A task is either
-//   * up to 4096 / n whole single-pass items of one size n <= 2^12, or
-//   * one whole two-pass item (2^13..2^20 points), both passes run by the one workgroup that
-//     claimed it, tile after tile (C = 4096 / N_f columns per tile).
-// No task depends on another: a workgroup only ever waits at its own barriers, and global data
-// written in a two-pass item's first pass is read back only by the same workgroup, so the
-// workgroup barrier orders it (no device-scope fences, which on this chip write back and
-// invalidate whole L2 caches). The host puts the largest items first, so the long tasks start
-// early and the queue ends with short ones.
+// (offset, log2 n, direction). A task is up to 4096 / n whole single-pass items of one size
+// n <= 2^12, one whole two-pass item below the split size (both passes by one workgroup, whose
+// barriers order them), or a run of tiles (C = 4096 / N_f columns each) of one pass of a larger
+// two-pass item, so a large item is spread over many workgroups (kernels.hpp).
+//
+// The queue runs in two phases: single-pass, whole and first-pass tasks, then second-pass runs,
+// which read what the first passes wrote. Device-scope release and acquire on this chip write
+// back and invalidate a whole XCD's L2, so they happen once per workgroup, not per task: when a
+// workgroup claims past phase 1 it publishes its phase-1 results (every wave writes back, then one
+// atomic adds the number of phase-1 tasks it ran), and before its first phase-2 task it waits
+// until all phase-1 tasks are published, then drops its stale cached lines. Only tasks already
+// claimed, hence held by running workgroups that will finish them without waiting, are waited for,
+// so the launch needs no co-residency of its workgroups. The wait is bounded: past the limit it
+// sets an error flag the host reports, and goes on. The host puts the largest items first within
+// each phase, so the long runs start early and the queue ends with short ones.
 #include "fft_bodies.hpp"
 
 namespace onefft {
@@ -195,29 +201,25 @@ __device__ void pass_tile(const RaggedArgs &a, const RaggedSize &z, const cf *in
     column_fft(io, lf, a.tw32 + tw, a.tw64 + tw64, lds + c * pass_span((int)lf), j);
 }
 
-// a whole two-pass item: every first-pass tile, a barrier, every second-pass tile
-__device__ void two_pass_item(const RaggedArgs &a, const RaggedTask &tk, uint32_t log2n, bool inv, cf *lds,
-                              uint32_t t) {
+// a run of tiles of one pass of a two-pass item
+__device__ void pass_tiles(const RaggedArgs &a, const RaggedTask &tk, uint32_t log2n, bool inv, bool first, cf *lds,
+                           uint32_t t) {
     const RaggedSize &z = a.size[log2n];
     const uint64_t off = a.item_off[tk.item];
-    const uint32_t tiles = 1u << (log2n - 12);  // tiles per pass
+    const uint32_t run = ((tk.word >> 8) & 255u) + 1;
     // The table offsets go through an empty asm statement each tile, so the compiler treats them
     // as new values and reloads the twiddles per tile instead of holding every tile's twiddles in
     // registers across the loop (which would cost occupancy).
-    for (uint32_t tile = 0; tile < tiles; tile++) {
-        uint32_t tw = tk.tw, tw64 = z.tw64_2;
+    for (uint32_t i = 0; i < run; i++) {
+        uint32_t tw = first ? z.tw2 : z.tw1, tw64 = first ? z.tw64_2 : z.tw64_1;
         asm volatile("" : "+v"(tw), "+v"(tw64));
-        pass_tile(a, z, a.in + off, a.out + off, tw, tw64, true, tile, inv, lds, t);
+        pass_tile(a, z, a.in + off, a.out + off, tw, tw64, first, tk.arg + i, inv, lds, t);
         __syncthreads();  // LDS reuse
     }
-    // the first pass's global stores, made by this workgroup, are complete and visible to it
-    for (uint32_t tile = 0; tile < tiles; tile++) {
-        uint32_t tw = tk.arg, tw64 = z.tw64_1;
-        asm volatile("" : "+v"(tw), "+v"(tw64));
-        pass_tile(a, z, a.in + off, a.out + off, tw, tw64, false, tile, inv, lds, t);
-        __syncthreads();
-    }
 }
+
+// bounded wait for the phase-1 results: about 3 s of 0.2 us sleeps
+constexpr uint32_t kRaggedSpinLimit = 1u << 24;
 
 }  // namespace
 
@@ -225,26 +227,56 @@ __global__ __launch_bounds__(kRaggedWG) void fft_ragged_kernel(RaggedArgs a) {
     __shared__ cf lds[kLds];
     __shared__ uint32_t claimed;
     const uint32_t t = threadIdx.x;
+    uint32_t mine = 0;       // phase-1 tasks this workgroup ran
+    bool published = false;  // its phase-1 results are published
     for (;;) {
         if (t == 0) claimed = __hip_atomic_fetch_add(a.head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
         const uint32_t id = __builtin_amdgcn_readfirstlane(claimed);  // uniform: task fields go to SGPRs
         __syncthreads();  // everyone has read `claimed` before it is overwritten
+        if (id >= a.nphase1 && !published) {
+            // phase 1 is claimed out: every wave writes its stores back, then one atomic publishes
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __syncthreads();
+            if (t == 0) __hip_atomic_fetch_add(a.head + 1, mine, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            published = true;
+            if (id < a.ntasks) {
+                // a second-pass task: wait until every phase-1 task is published
+                if (t == 0) {
+                    for (uint32_t spins = 0;
+                         __hip_atomic_load(a.head + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < a.nphase1;
+                         spins++) {
+                        if (spins >= kRaggedSpinLimit) {
+                            __hip_atomic_store(a.head + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(8);
+                    }
+                }
+                __syncthreads();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // every wave drops stale cached lines
+            }
+        }
         if (id >= a.ntasks) break;
         const RaggedTask tk = a.tasks[id];
         const uint32_t kind = tk.word & 3u, log2n = (tk.word >> 2) & 31u;
         const bool inv = (tk.word >> 7) & 1u;
         if (kind == kRaggedSingle) {
             single_task(a, tk, log2n, inv, lds, t);
+        } else if (kind == kRaggedWhole) {
+            pass_tiles(a, tk, log2n, inv, true, lds, t);
+            // the first pass's global stores, made by this workgroup, are visible to it here
+            pass_tiles(a, tk, log2n, inv, false, lds, t);
         } else {
-            two_pass_item(a, tk, log2n, inv, lds, t);
+            pass_tiles(a, tk, log2n, inv, kind == kRaggedFirst, lds, t);
         }
+        if (id < a.nphase1) mine++;
         __syncthreads();  // LDS is free for the next task
     }
 }
 
 __global__ __launch_bounds__(64) void fft_ragged_reset(uint32_t *head) {
-    if (threadIdx.x == 0) *head = 0;
+    if (threadIdx.x < 3) head[threadIdx.x] = 0;
 }
 
 uint32_t ragged_resident_grid() {

@@ -95,23 +95,32 @@ hipError_t launch_pass(int log2nf, int cols, bool first, bool swap_in, bool swap
 
 // ---- ragged batches: one persistent launch over items of mixed sizes -------------------------
 //
-// A task is either up to 4096 / n whole transforms of one size n <= 2^12 (single-pass), or one
-// whole two-pass transform of 2^13..2^20 points, run by one 256-thread workgroup in tiles of
-// 4096 points (C = 4096 / N_f columns of an N_f-point sub-transform).
+// A task is one of
+//   * up to 4096 / n whole transforms of one size n <= 2^12 (single-pass);
+//   * one whole two-pass transform of 2^13 points up to below the split size (both passes, tile
+//     by tile: a tile is 4096 points, C = 4096 / N_f columns of an N_f-point sub-transform), run
+//     by one workgroup, whose second pass then reads its first pass's output from cache;
+//   * a run of up to kRaggedTileRun tiles of the first pass of a two-pass transform of the split
+//     size or larger (up to 2^20 points), so one large transform is spread over many workgroups;
+//   * a run of tiles of such a transform's second pass.
+// The queue holds the single-pass, whole and first-pass tasks (phase 1) before the second-pass
+// runs (phase 2); a workgroup starts its first phase-2 task only once every phase-1 task's
+// results are published device-wide.
 constexpr int kRaggedMaxLog2 = 20;
 constexpr int kRaggedSingleMaxLog2 = 12;
 constexpr int kRaggedWG = 256;
 constexpr uint32_t kRaggedTaskPoints = 4096;
+constexpr uint32_t kRaggedTileRun = 16;  // tiles per pass task (at most 256)
+constexpr int kRaggedSplitLog2 = 19;     // default split size (ONEFFT_RAGGED_SPLIT overrides)
 
-enum RaggedKind : uint32_t { kRaggedSingle = 0, kRaggedTwoPass = 1 };
+enum RaggedKind : uint32_t { kRaggedSingle = 0, kRaggedFirst = 1, kRaggedSecond = 2, kRaggedWhole = 3 };
 
 struct RaggedTask {
-    uint32_t word;  // kind | log2n << 2 | inverse << 7
-    uint32_t item;  // single: first index into the sorted item list; two-pass: the item's index
-    uint32_t arg;   // single: number of transforms; two-pass: stage table of N_1 (second pass)
-    uint32_t tw;    // stage table this task reads first (single: of n; two-pass: of N_2), as an
-                    // offset in the FP32 table blob, read per task so the compiler cannot hoist
-                    // twiddle loads out of the task loop
+    uint32_t word;  // kind | log2n << 2 | inverse << 7 | (tiles - 1) << 8
+    uint32_t item;  // single: first index into the sorted item list; others: the item's index
+    uint32_t arg;   // single: number of transforms; pass runs: the first tile; whole: 0
+    uint32_t tw;    // single: the stage table of n, as an offset in the FP32 table blob, read per
+                    // task so the compiler cannot hoist twiddle loads out of the task loop
 };
 
 struct RaggedSize {
@@ -129,15 +138,16 @@ struct RaggedArgs {
     cf *out;
     const RaggedTask *tasks;
     const uint64_t *item_off;  // complex offset of each item, in task order
-    uint32_t *head;            // next task to claim
+    uint32_t *head;            // [0] next task to claim, [1] phase-1 tasks published, [2] error flag
     const cf *tw32;
     const cd *tw64;
     uint32_t ntasks;
+    uint32_t nphase1;          // tasks [0, nphase1) are phase 1
     uint32_t nitems;
     RaggedSize size[kRaggedMaxLog2 + 1];
 };
 
-// Zeroes the queue head (with a kernel, so the pair can be captured in a graph), then runs the
+// Zeroes the queue counters (with a kernel, so the pair can be captured in a graph), then runs the
 // persistent kernel on `grid` workgroups (0: as many as fit the device at once).
 hipError_t launch_ragged(const RaggedArgs &a, uint32_t grid, hipStream_t stream);
 // Workgroups of the persistent kernel that fit the device at once.

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -158,12 +159,35 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
     for (size_t k = 0; k < count; k++) item_off[k] = items[order[k]].offset;
 
     std::vector<RaggedTask> tasks;
-    auto word = [](uint32_t kind, uint32_t log2n, uint32_t inv) { return kind | (log2n << 2) | (inv << 7); };
+    auto word = [](uint32_t kind, uint32_t log2n, uint32_t inv, uint32_t tiles = 1) {
+        return kind | (log2n << 2) | (inv << 7) | ((tiles - 1) << 8);
+    };
+    // two-pass items below the split size run whole on one workgroup; larger ones are split into
+    // runs of tiles over both phases (ONEFFT_RAGGED_SPLIT: the split size's log2, for measurement)
+    uint32_t split = (uint32_t)kRaggedSplitLog2;
+    if (const char *env = std::getenv("ONEFFT_RAGGED_SPLIT")) {
+        const int v = std::atoi(env);
+        if (v > kRaggedSingleMaxLog2 && v <= kRaggedMaxLog2 + 1) split = (uint32_t)v;
+    }
+    auto pass_tasks = [&](uint32_t kind) {
+        for (size_t k = 0; k < nbig; k++) {
+            const onefft_ragged_item &it = items[order[k]];
+            if (it.log2n < split) continue;
+            const uint32_t tiles = 1u << (it.log2n - 12);  // 4096-point tiles per pass
+            for (uint32_t t0 = 0; t0 < tiles; t0 += kRaggedTileRun) {
+                const uint32_t run = tiles - t0 < kRaggedTileRun ? tiles - t0 : kRaggedTileRun;
+                tasks.push_back({word(kind, it.log2n, it.inverse, run), (uint32_t)k, t0, 0});
+            }
+        }
+    };
+    // phase 1: the whole two-pass items (the longest tasks, largest first), the first passes of
+    // the split items, then the single-pass items grouped by size (largest first) and direction
     for (size_t k = 0; k < nbig; k++) {
         const onefft_ragged_item &it = items[order[k]];
-        const RaggedSize &z = a.size[it.log2n];
-        tasks.push_back({word(kRaggedTwoPass, it.log2n, it.inverse), (uint32_t)k, z.tw1, z.tw2});
+        if (it.log2n < split)
+            tasks.push_back({word(kRaggedWhole, it.log2n, it.inverse, 1u << (it.log2n - 12)), (uint32_t)k, 0, 0});
     }
+    pass_tasks(kRaggedFirst);
     for (size_t k = nbig; k < count;) {
         const onefft_ragged_item &it = items[order[k]];
         size_t e = k;
@@ -173,6 +197,9 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
         tasks.push_back({word(kRaggedSingle, it.log2n, it.inverse), (uint32_t)k, (uint32_t)(e - k), a.size[it.log2n].tw});
         k = e;
     }
+    const size_t nphase1 = tasks.size();
+    // phase 2: the second passes
+    pass_tasks(kRaggedSecond);
     if (tasks.size() > 0xffffffffu) {
         set_error("too many tasks");
         delete plan;
@@ -184,7 +211,7 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
     const size_t o_tasks = 0;
     const size_t o_off = up(o_tasks + tasks.size() * sizeof(RaggedTask));
     const size_t o_head = up(o_off + item_off.size() * sizeof(uint64_t));
-    const size_t o_tw32 = up(o_head + sizeof(uint32_t));
+    const size_t o_tw32 = up(o_head + 4 * sizeof(uint32_t));
     const size_t o_tw64 = up(o_tw32 + tw32.size() * sizeof(cf));
     const size_t bytes = up(o_tw64 + tw64.size() * sizeof(cd));
     std::vector<unsigned char> image(bytes, 0);
@@ -212,6 +239,7 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
     a.tw32 = reinterpret_cast<const cf *>(d + o_tw32);
     a.tw64 = reinterpret_cast<const cd *>(d + o_tw64);
     a.ntasks = (uint32_t)tasks.size();
+    a.nphase1 = (uint32_t)nphase1;
     a.nitems = (uint32_t)count;
     plan->grid = ragged_resident_grid();
     // the per-item comparison uses the cached plan of every size
@@ -282,6 +310,17 @@ extern "C" int onefft_ragged_wait(const onefft_ragged_plan *plan, struct ihipStr
     hipError_t e = hipStreamSynchronize(stream);
     if (e != hipSuccess) {
         set_error("ragged execution", e);
+        return -1;
+    }
+    // the kernel's bounded wait for phase 1 sets head[2] if it ever gives up
+    uint32_t flag = 0;
+    e = hipMemcpy(&flag, plan->args.head + 2, sizeof(flag), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        set_error("ragged status read", e);
+        return -1;
+    }
+    if (flag) {
+        set_error("ragged execution: the wait for the first passes timed out");
         return -1;
     }
     return 0;

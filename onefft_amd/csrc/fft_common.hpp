@@ -4,7 +4,9 @@
 // Conventions (the reference's, include/onefft.h:26-34): data are interleaved complex float
 // (re, im), forward is exp(-2 pi i jk/n), nothing is normalised. The kernels only implement the
 // forward direction; the inverse exchanges re and im on the way in and out, which is the
-// reference's own inverse (src/onefft.c:86-88, docs/design.md:137-144).
+// reference's own inverse (src/onefft.c:86-88, docs/design.md:137-144), or, in the single-pass
+// kernels, conjugates on the way in and out (the same transform: swap(F(swap x)) =
+// conj(F(conj x))).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -12,7 +14,7 @@
 
 #include <type_traits>
 
-// Register-level helpers are also host functions, so the butterflies can be checked on a CPU.
+// Helpers are host-callable too: the plan builder includes this header for the stage schedule.
 #define ONEFFT_HD __host__ __device__ __forceinline__
 
 namespace onefft {

@@ -218,9 +218,6 @@ __device__ void pass_tiles(const RaggedArgs &a, const RaggedTask &tk, uint32_t l
     }
 }
 
-// bounded wait for the phase-1 results: about 3 s of 0.2 us sleeps
-constexpr uint32_t kRaggedSpinLimit = 1u << 24;
-
 }  // namespace
 
 __global__ __launch_bounds__(kRaggedWG) void fft_ragged_kernel(RaggedArgs a) {
@@ -246,7 +243,7 @@ __global__ __launch_bounds__(kRaggedWG) void fft_ragged_kernel(RaggedArgs a) {
                     for (uint32_t spins = 0;
                          __hip_atomic_load(a.head + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < a.nphase1;
                          spins++) {
-                        if (spins >= kRaggedSpinLimit) {
+                        if (spins >= a.spin_limit) {
                             __hip_atomic_store(a.head + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                             break;
                         }

@@ -144,6 +144,7 @@ struct RaggedArgs {
     uint32_t ntasks;
     uint32_t nphase1;          // tasks [0, nphase1) are phase 1
     uint32_t nitems;
+    uint32_t spin_limit;       // sleeps of the bounded wait for phase 1 before it gives up
     RaggedSize size[kRaggedMaxLog2 + 1];
 };
 

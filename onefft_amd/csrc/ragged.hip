@@ -241,6 +241,10 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
     a.ntasks = (uint32_t)tasks.size();
     a.nphase1 = (uint32_t)nphase1;
     a.nitems = (uint32_t)count;
+    // the bounded wait for phase 1: ~3 s of 0.2 us sleeps (ONEFFT_RAGGED_SPIN_LIMIT overrides it,
+    // so tests can make the wait give up and check that the failure is reported)
+    a.spin_limit = 1u << 24;
+    if (const char *env = std::getenv("ONEFFT_RAGGED_SPIN_LIMIT")) a.spin_limit = (uint32_t)std::strtoul(env, nullptr, 10);
     plan->grid = ragged_resident_grid();
     // the per-item comparison uses the cached plan of every size
     for (int l = 0; l <= kRaggedMaxLog2; l++)

@@ -120,3 +120,23 @@ def test_workload_prefix_against_reference():
         off, l, inv = items[k]
         seg = slice(2 * off, 2 * (off + (1 << l)))
         assert O.rel_rms(y[seg], O.fft_f64(x[seg], 1 << l, bool(inv))) < tol_fp64(l)
+
+
+def test_phase_wait_timeout_is_reported(monkeypatch):
+    """Failure handling, forced from the host: a plan whose bounded wait for the first phase gives
+    up at once (limit 0) still runs the launch to its end (every access stays inside the items),
+    and onefft_ragged_wait reports the timeout; a default plan of the same items is then correct."""
+    items, extent = layout([19] + [10] * 200, seed=5)  # the 2^19 item's second passes wait
+    x = O.normal(33, extent)
+    d_in = torch.from_numpy(x).cuda()
+    d_out = torch.zeros(2 * extent, device="cuda")
+    monkeypatch.setenv("ONEFFT_RAGGED_SPIN_LIMIT", "0")
+    p = F.RaggedPlan(items)
+    monkeypatch.delenv("ONEFFT_RAGGED_SPIN_LIMIT")
+    p.execute(d_in.data_ptr(), d_out.data_ptr(), stream())
+    with pytest.raises(F.OneFFTError, match="timed out"):
+        p.wait(stream())
+    p.close()
+    y, q = run(items, extent, x)
+    check_items(items, x, y, extent)
+    q.close()

@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the timed transform launches plus one 1 GiB device copy (for counter runs)")
     ap.add_argument("--no-extra", action="store_true", help="skip the other workloads")
+    ap.add_argument("--single-transform", type=int, default=None, metavar="LOG2N",
+                    help="instead of the benchmark: 3 forward transforms of one 2^LOG2N-point input (random, "
+                         "resident in HBM) and one device copy of the same bytes, for counter runs")
     ap.add_argument("--report", action="store_true",
                     help="add a per-configuration report: errors against the reference CPU transform and the "
                          "FP64 oracle, and the reference's CPU times (several minutes of host work)")
@@ -444,6 +447,19 @@ def main():
 
     if args.variants:
         variants_ab(torch, F, [int(v) for v in args.variants.split(",")])
+        return
+    if args.single_transform is not None:
+        n = 1 << args.single_transform
+        x = torch.randn(n, dtype=torch.complex64, device="cuda")
+        y = torch.empty_like(x)
+        p = F.Plan(n)
+        sp = torch.cuda.current_stream().cuda_stream
+        for _ in range(3):
+            p.batch_ptr(x.data_ptr(), y.data_ptr(), 1, n, n, False, sp)
+        z = torch.empty_like(x)
+        z.copy_(x)  # a plain copy of known size: 8 n bytes read, 8 n bytes written
+        torch.cuda.synchronize()
+        print(json.dumps({"single_transform": args.single_transform, "launches": 3}), flush=True)
         return
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)

@@ -69,11 +69,13 @@ def time_ms(torch, stream, fn, reps):
 
 
 def variants_ab(torch, F, sizes):
-    """Interleaved A/B of the single-pass kernel variants (ONEFFT_BATCH_VARIANT = "nt,layout,tw64")."""
+    """Interleaved A/B of the single-pass kernel variants (ONEFFT_BATCH_VARIANT = "nt,layout,tw64", tw64 a
+    level 0..3)."""
     import statistics
 
     variants = {"padded": "0,0,0", "padded_nt": "1,0,0", "swizzled": "0,1,0", "swizzled_nt": "1,1,0",
-                "padded_tw64": "0,0,1", "padded_nt_tw64": "1,0,1"}
+                "padded_tw64": "0,0,1", "padded_nt_tw64": "1,0,1", "padded_nt_tw64_ns16": "1,0,2",
+                "padded_nt_tw64_all": "1,0,3"}
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
     old = os.environ.get("ONEFFT_BATCH_VARIANT")

@@ -19,7 +19,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libonefft_amd.so")
-SOURCES = ["runtime.hip", "staging.hip", "ragged.hip", "graph.hip", "tuning.hip", "k_batch.hip", "k_pass.hip", "k_ragged.hip"]
+SOURCES = ["runtime.hip", "staging.hip", "ragged.hip", "graph.hip", "tuning.hip", "k_batch.hip", "k_batch_tw.hip", "k_pass.hip",
+           "k_ragged.hip"]
 ARCH = "gfx950"
 
 

@@ -47,7 +47,7 @@ __device__ __forceinline__ void single_load(const IO &io, uint32_t j, cf (&v)[Si
     });
 }
 
-template <int LOG2N, bool INV, int LAYOUT, bool TW64, class IO>
+template <int LOG2N, bool INV, int LAYOUT, int TW64, class IO>
 __device__ __forceinline__ void single_finish(cf (&v)[Single<LOG2N>::P], const IO &io, bool live,
                                               const cf *__restrict__ tw, cf *L, uint32_t j,
                                               const cd *__restrict__ tw64) {
@@ -64,7 +64,7 @@ __device__ __forceinline__ void single_finish(cf (&v)[Single<LOG2N>::P], const I
         constexpr int BF = P / R;  // butterflies per thread
         constexpr uint32_t STRIDE = N / R;
         constexpr uint32_t TWO = stage_tw_offset(LOG2N, LOG2P, s);
-        constexpr bool BASE64 = TW64 && stage_uses_tw64(LOG2N, LOG2P, s);
+        constexpr bool BASE64 = stage_uses_tw64(LOG2N, LOG2P, s, TW64);
         constexpr uint32_t TWO64 = stage_tw64_offset(LOG2N, LOG2P, s);
 
         if constexpr (s > 0) {
@@ -126,7 +126,7 @@ __device__ __forceinline__ void single_finish(cf (&v)[Single<LOG2N>::P], const I
 }
 
 // `live` false: the loads still run (the caller points io at a valid transform), the stores do not
-template <int LOG2N, bool INV, int LAYOUT, bool TW64, class IO>
+template <int LOG2N, bool INV, int LAYOUT, int TW64, class IO>
 __device__ __forceinline__ void single_body(const IO &io, bool live, const cf *__restrict__ tw, cf *L, uint32_t j,
                                             const cd *__restrict__ tw64) {
     cf v[Single<LOG2N>::P];
@@ -229,7 +229,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, const cf *__restric
             constexpr int qq = decltype(q)::value;
             if constexpr (s > 0) {
                 const uint32_t k = (j + T * qq) & (NS - 1);
-                if constexpr (stage_uses_tw64(LOG2NF, LOG2P, s)) {
+                if constexpr (stage_uses_tw64(LOG2NF, LOG2P, s, 1)) {
                     // w^(r k) by an FP64 recurrence from the base w^k, each rounded once
                     const cd base = a.tw64[stage_tw64_offset(LOG2NF, LOG2P, s) + k];
                     cd w = base;

@@ -203,21 +203,38 @@ __host__ __device__ constexpr uint32_t stage_tw_size(int log2n, int log2p) {
     return stage_tw_offset(log2n, log2p, stage_count(log2n, log2p));
 }
 
-// Stages whose butterflies read distinct twiddles in every lane (Ns >= 64) can instead take one
-// FP64 base w_{Ns R}^k per butterfly and form w^(r k), r = 2..R-1, by an FP64 recurrence, each
-// rounded once to FP32: 16 bytes of table per butterfly instead of 8 (R - 1). The FP64 base table
-// of stage s holds w_{Ns R}^k for k < Ns at stage_tw64_offset(.., s).
-__host__ __device__ constexpr bool stage_uses_tw64(int log2n, int log2p, int s) {
-    return s > 0 && stage_log2ns(log2n, log2p, s) >= 6 && stage_log2r(log2n, log2p, s) >= 2;
+// Stages after the first with radix >= 4 can take one FP64 base w_{Ns R}^k per butterfly instead
+// of R - 1 table twiddles and form w^(r k), r = 2..R-1, by an FP64 recurrence, each rounded once
+// to FP32: 16 bytes of table per butterfly instead of 8 (R - 1), at ~6 FP64 instructions per
+// twiddle. Plans hold the FP64 bases of every such stage (w_{Ns R}^k for k < Ns at
+// stage_tw64_offset(.., s)); a kernel uses them for the stages with Ns >= 2^tw64_min_log2ns(level),
+// the level being a tuning choice: 0 none, 1 Ns >= 64 (the stages whose FP32 tables outgrow the
+// L1 cache), 2 Ns >= 16, 3 every such stage.
+__host__ __device__ constexpr bool stage_has_tw64(int log2n, int log2p, int s) {
+    return s > 0 && stage_log2r(log2n, log2p, s) >= 2;
+}
+__host__ __device__ constexpr int tw64_min_log2ns(int level) {
+    return level <= 0 ? 64 : level == 1 ? 6 : level == 2 ? 4 : 0;
+}
+__host__ __device__ constexpr bool stage_uses_tw64(int log2n, int log2p, int s, int level) {
+    return stage_has_tw64(log2n, log2p, s) && stage_log2ns(log2n, log2p, s) >= tw64_min_log2ns(level);
 }
 __host__ __device__ constexpr uint32_t stage_tw64_offset(int log2n, int log2p, int s) {
     uint32_t off = 0;
     for (int i = 1; i < s; i++)
-        if (stage_uses_tw64(log2n, log2p, i)) off += 1u << stage_log2ns(log2n, log2p, i);
+        if (stage_has_tw64(log2n, log2p, i)) off += 1u << stage_log2ns(log2n, log2p, i);
     return off;
 }
 __host__ __device__ constexpr uint32_t stage_tw64_size(int log2n, int log2p) {
     return stage_tw64_offset(log2n, log2p, stage_count(log2n, log2p));
+}
+// the highest level that changes the kernel of this size (0: no stage can use FP64 bases)
+__host__ __device__ constexpr int tw64_levels(int log2n, int log2p) {
+    int top = 0;
+    for (int level = 1; level <= 3; level++)
+        for (int s = 1; s < stage_count(log2n, log2p); s++)
+            if (stage_uses_tw64(log2n, log2p, s, level) && !stage_uses_tw64(log2n, log2p, s, level - 1)) top = level;
+    return top;
 }
 
 // ---- LDS layout ---------------------------------------------------------------------------

@@ -9,50 +9,9 @@
 // order without a bit-reversal pass). The inverse is conj(F(conj x)), the same transform as the
 // reference's re/im exchange around F (src/onefft.c:86-88) up to rounding: the negations fold
 // into the first and last butterflies' additions, where the exchange cost register moves.
-#include "fft_bodies.hpp"
+#include "k_batch.hpp"
 
 namespace onefft {
-
-template <int LOG2N, bool INV, bool NT, int LAYOUT, bool TW64>
-__global__ __launch_bounds__(batch_wg(LOG2N)) void fft_batch_kernel(const cf *__restrict__ in, cf *__restrict__ out,
-                                                                      const cf *__restrict__ tw,
-                                                                      const cd *__restrict__ tw64, uint64_t in_dist,
-                                                                      uint64_t out_dist, uint64_t batch) {
-    constexpr int N = 1 << LOG2N;
-    constexpr uint32_t T = N >> batch_log2p(LOG2N);  // threads per transform
-    constexpr int PER = batch_per_wg(LOG2N);         // transforms per workgroup
-    constexpr bool USES_LDS = stage_count(LOG2N, batch_log2p(LOG2N)) > 1;
-    constexpr uint32_t SPAN = lds_extent<LAYOUT>(N);
-    static_assert(batch_wg(LOG2N) % T == 0, "workgroup must hold whole transforms");
-    __shared__ cf lds[USES_LDS ? PER * SPAN : 1];
-
-    const uint32_t t = threadIdx.x;
-    if constexpr (PER == 1) {
-        // one transform per workgroup, and the grid has exactly one workgroup per transform: its
-        // base is uniform, so it goes into buffer descriptors
-        const uint64_t which = blockIdx.x;
-        const BufIO<NT> io(in + which * in_dist, out + which * out_dist);
-        single_body<LOG2N, INV, LAYOUT, TW64>(io, true, tw, lds, t, tw64);
-    } else {
-        // the last workgroup's lanes past the batch load the batch's last transform again and
-        // store nothing
-        const uint32_t c = t / T;
-        const uint64_t which = (uint64_t)blockIdx.x * PER + c;
-        const uint64_t src = which < batch ? which : batch - 1;
-        const PtrIO<NT> io{in + src * in_dist, out + which * out_dist};
-        single_body<LOG2N, INV, LAYOUT, TW64>(io, which < batch, tw, lds + (USES_LDS ? c * SPAN : 0), t % T, tw64);
-    }
-}
-
-template <int LOG2N, bool NT, int LAYOUT, bool TW64 = false>
-static void launch_variant(bool inverse, const BatchArgs &a, dim3 grid, hipStream_t stream) {
-    if (inverse)
-        hipLaunchKernelGGL((fft_batch_kernel<LOG2N, true, NT, LAYOUT, TW64>), grid, dim3(batch_wg(LOG2N)), 0, stream,
-                           a.in, a.out, a.tw, a.tw64, a.in_dist, a.out_dist, a.batch);
-    else
-        hipLaunchKernelGGL((fft_batch_kernel<LOG2N, false, NT, LAYOUT, TW64>), grid, dim3(batch_wg(LOG2N)), 0, stream,
-                           a.in, a.out, a.tw, a.tw64, a.in_dist, a.out_dist, a.batch);
-}
 
 template <int LOG2N>
 static hipError_t launch_batch_n(bool inverse, const BatchArgs &a, BatchVariant v, hipStream_t stream) {
@@ -62,14 +21,11 @@ static hipError_t launch_batch_n(bool inverse, const BatchArgs &a, BatchVariant 
     if (blocks > 0x7fffffffu) return hipErrorInvalidValue;
     const dim3 grid((uint32_t)blocks);
     constexpr bool USES_LDS = stage_count(LOG2N, batch_log2p(LOG2N)) > 1;
-    constexpr bool HAS_TW64 = stage_tw64_size(LOG2N, batch_log2p(LOG2N)) > 0;
-    if constexpr (HAS_TW64) {
-        if (v.tw64) {  // padded layout only
-            if (v.nt) launch_variant<LOG2N, true, kPadded, true>(inverse, a, grid, stream);
-            else launch_variant<LOG2N, false, kPadded, true>(inverse, a, grid, stream);
-            return hipGetLastError();
-        }
-    }
+    // FP64-base levels (fft_common.hpp: stage_uses_tw64), padded layout only; a level above the
+    // last one that changes this size's kernel runs that one (k_batch_tw.hip)
+    constexpr int TOP = tw64_levels(LOG2N, batch_log2p(LOG2N));
+    const int level = v.tw64 < TOP ? v.tw64 : TOP;
+    if (level > 0) return launch_batch_tw64(LOG2N, level, v.nt, inverse, a, grid, stream);
     const bool swz = USES_LDS && v.layout == kSwizzled;
     if (v.nt) {
         if (swz) launch_variant<LOG2N, true, kSwizzled>(inverse, a, grid, stream);

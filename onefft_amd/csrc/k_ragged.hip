@@ -118,7 +118,7 @@ __device__ void column_fft(const Column &io, uint32_t log2n, const cf *tw, const
         const bool last = log2ns + 4 == log2n;
         static_for<0, 16>([&](auto r) { v[r] = L[lds_pad(j + decltype(r)::value * stride)]; });
         const uint32_t k = j & (NS - 1);
-        if (log2ns >= 6) {
+        if (log2ns >= (uint32_t)tw64_min_log2ns(1)) {
             // wide stage: w^(r k) by an FP64 recurrence from the base w^k, each rounded once
             const cd base = tw64[off64 + k];
             cd w = base;
@@ -126,7 +126,6 @@ __device__ void column_fft(const Column &io, uint32_t log2n, const cf *tw, const
                 v[r] = mul(v[r], round32(w));
                 if constexpr (decltype(r)::value + 1 < 16) w = mul64(w, base);
             });
-            off64 += NS;
         } else {
             static_for<1, 16>([&](auto r) {
                 constexpr int rr = decltype(r)::value;
@@ -145,6 +144,7 @@ __device__ void column_fft(const Column &io, uint32_t log2n, const cf *tw, const
         static_for<0, 16>([&](auto m) { L[lds_pad(base + decltype(m)::value * NS)] = v[m]; });
         __syncthreads();
         off += 15 * NS;
+        off64 += NS;  // the FP64 bases of every radix-16 stage are in the table, used or not
         log2ns += 4;
     }
 }

@@ -35,13 +35,13 @@ struct BatchArgs {
 };
 
 // Kernel variant of the single-pass transform: nontemporal global access, the LDS layout
-// (fft_common.hpp: kPadded / kSwizzled), and FP64-base twiddles for the wide stages
-// (fft_common.hpp: stage_uses_tw64; with the padded layout only). The plan's tables serve every
+// (fft_common.hpp: kPadded / kSwizzled), and the level of FP64-base twiddles (0..3,
+// fft_common.hpp: stage_uses_tw64; with the padded layout only). The plan's tables serve every
 // variant.
 struct BatchVariant {
     bool nt = true;
     int layout = 0;
-    bool tw64 = true;
+    int tw64 = 1;
 };
 
 // One launch: `batch` transforms of 2^log2n points, 0 <= log2n <= kMaxSinglePassLog2.

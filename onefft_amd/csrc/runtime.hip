@@ -83,11 +83,11 @@ void fill_stage_table(int log2n, int log2p, cf *table) {
     }
 }
 
-// FP64 bases w_{Ns R}^k, k < Ns, of the stages that use them (fft_common.hpp: stage_uses_tw64).
+// FP64 bases w_{Ns R}^k, k < Ns, of every stage that can use them (fft_common.hpp: stage_has_tw64).
 void fill_stage_table64(int log2n, int log2p, cd *table) {
     const int stages = stage_count(log2n, log2p);
     for (int s = 1; s < stages; s++) {
-        if (!stage_uses_tw64(log2n, log2p, s)) continue;
+        if (!stage_has_tw64(log2n, log2p, s)) continue;
         const uint32_t off = stage_tw64_offset(log2n, log2p, s);
         const int lr = stage_log2r(log2n, log2p, s);
         const int lns = stage_log2ns(log2n, log2p, s);
@@ -183,7 +183,7 @@ onefft::BatchVariant choose_variant(unsigned log2n, int regime) {
         if (std::sscanf(env, "%d,%d,%d", &nt, &layout, &tw64) >= 2) {
             v.nt = nt != 0;
             v.layout = layout ? 1 : 0;
-            v.tw64 = tw64 != 0;
+            v.tw64 = tw64 < 0 ? 0 : tw64 > 3 ? 3 : tw64;
         }
     }
     return v;

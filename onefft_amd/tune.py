@@ -4,7 +4,8 @@ tuning table the library reads at plan creation (tuning.hip).
     python -m onefft_amd.tune [--min 0] [--max 30] [--out onefft_amd/tune_gfx950.tsv]
 
 * sizes up to 2^14 (one LDS pass): the single-pass variants (nontemporal global access on or off,
-  padded or swizzled LDS, and from 2^10 FP64-base twiddles for the wide stages);
+  padded or swizzled LDS, and, where the size has such stages, FP64-base twiddles at levels 1..3:
+  for the stages with Ns >= 64, >= 16, or all);
 * larger sizes: every split n = N_1 ... N_p into 2..4 passes with factors 2^6..2^10 (4 passes only
   from 2^24), each with the default and the widest allowed columns per workgroup;
 each in two working-set regimes, because the best choice differs between them: a batch whose
@@ -51,6 +52,23 @@ def widest_cols(log2nf: int) -> int:
     return max(c for c in (8, 16, 32, 64) if pass_cols_ok(log2nf, c))
 
 
+def tw64_levels(log2n: int):
+    """The FP64-base levels that give distinct kernels for 2^log2n points (fft_common.hpp:
+    stage_uses_tw64 and tw64_levels, over the single-pass stage schedule)."""
+    p = min(log2n, 4)
+    stages = 1 if log2n <= p else log2n // p + (1 if log2n % p else 0)
+    log2r = [log2n if log2n <= p else (log2n % p if (log2n % p and s == 0) else p) for s in range(stages)]
+    log2ns = [sum(log2r[:s]) for s in range(stages)]
+    has = [s for s in range(1, stages) if log2r[s] >= 2]
+    out, used = [], set()
+    for level, min_log2ns in ((1, 6), (2, 4), (3, 0)):
+        now = {s for s in has if log2ns[s] >= min_log2ns}
+        if now != used:
+            out.append(level)
+        used = now
+    return out
+
+
 def splits(L: int):
     out = []
     for p in (2, 3, 4):
@@ -65,8 +83,8 @@ def splits(L: int):
 def candidates(L: int, single: bool):
     if single:
         combos = [(nt, lay, 0) for nt in (1, 0) for lay in (0, 1)]
-        if L >= 10:  # sizes with a stage wide enough for FP64-base twiddles (padded layout only)
-            combos += [(nt, 0, 1) for nt in (1, 0)]
+        # FP64-base levels that change this size's kernel (padded layout only)
+        combos += [(nt, 0, level) for level in tw64_levels(L) for nt in (1, 0)]
         return [dict(kind="single", params=f"nt={nt},layout={lay},tw64={t64}",
                      env={"ONEFFT_BATCH_VARIANT": f"{nt},{lay},{t64}"}) for nt, lay, t64 in combos]
     out = []

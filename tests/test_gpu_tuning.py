@@ -65,3 +65,37 @@ def test_committed_table_parses():
             continue
         log2n, kind, params, gbs, batch = line.rstrip("\n").split("\t")
         assert 0 <= int(log2n) <= 30 and kind in ("single", "multi") and float(gbs) > 0 and int(batch) >= 1
+
+
+def test_every_single_pass_variant(monkeypatch):
+    """Every kernel variant the tuner may choose, at every single-pass size, both directions,
+    against the FP64 oracle (variants are chosen at plan creation from ONEFFT_BATCH_VARIANT)."""
+    import numpy as np
+    import torch
+
+    import onefft_amd as F
+    from conftest import tol_fp64
+    from oracle import oracle as O
+    from onefft_amd.tune import tw64_levels
+
+    s = torch.cuda.current_stream().cuda_stream
+    for L in range(0, 15):
+        n = 1 << L
+        batch = 3
+        x = O.normal(60 + L, n * batch)
+        d = torch.from_numpy(x).cuda()
+        variants = [(nt, lay, 0) for nt in (0, 1) for lay in (0, 1)]
+        variants += [(nt, 0, lv) for lv in tw64_levels(L) for nt in (0, 1)]
+        for nt, lay, lv in variants:
+            monkeypatch.setenv("ONEFFT_BATCH_VARIANT", f"{nt},{lay},{lv}")
+            p = F.Plan(n)
+            for inv in (False, True):
+                o = torch.empty_like(d)
+                p.batch_ptr(d.data_ptr(), o.data_ptr(), batch, n, n, inv, s)
+                torch.cuda.synchronize()
+                got = o.cpu().numpy()
+                for b in range(batch):
+                    seg = slice(2 * n * b, 2 * n * (b + 1))
+                    e = O.rel_rms(got[seg], O.fft_f64(x[seg], n, inv))
+                    assert e < tol_fp64(L), (L, nt, lay, lv, inv, b, e)
+            p.close()

@@ -186,7 +186,6 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, const cf *__restric
         constexpr uint32_t NS = 1u << LNS;
         constexpr int BF = P / R;
         constexpr uint32_t STRIDE = NF / R;
-        constexpr uint32_t TWO = stage_tw_offset(LOG2NF, LOG2P, s);
 
         if constexpr (s == 0) {
             static_for<0, BF>([&](auto q) {
@@ -229,21 +228,16 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, const cf *__restric
             constexpr int qq = decltype(q)::value;
             if constexpr (s > 0) {
                 const uint32_t k = (j + T * qq) & (NS - 1);
-                if constexpr (stage_uses_tw64(LOG2NF, LOG2P, s, 1)) {
-                    // w^(r k) by an FP64 recurrence from the base w^k, each rounded once
-                    const cd base = a.tw64[stage_tw64_offset(LOG2NF, LOG2P, s) + k];
-                    cd w = base;
-                    static_for<1, R>([&](auto r) {
-                        constexpr int rr = decltype(r)::value;
-                        v[qq * R + rr] = mul(v[qq * R + rr], round32(w));
-                        if constexpr (rr + 1 < R) w = mul64(w, base);
-                    });
-                } else {
-                    static_for<1, R>([&](auto r) {
-                        constexpr int rr = decltype(r)::value;
-                        v[qq * R + rr] = mul(v[qq * R + rr], a.tw[TWO + (rr - 1) * NS + k]);
-                    });
-                }
+                // every stage after the first has radix 16: w^(r k) by an FP64 recurrence from the
+                // base w^k, each rounded once (FP64-base level 3, fft_common.hpp: stage_uses_tw64)
+                static_assert(stage_uses_tw64(LOG2NF, LOG2P, s, 3), "pass stages after the first have FP64 bases");
+                const cd base = a.tw64[stage_tw64_offset(LOG2NF, LOG2P, s) + k];
+                cd w = base;
+                static_for<1, R>([&](auto r) {
+                    constexpr int rr = decltype(r)::value;
+                    v[qq * R + rr] = mul(v[qq * R + rr], round32(w));
+                    if constexpr (rr + 1 < R) w = mul64(w, base);
+                });
             }
             dft<R>(&v[qq * R]);
         });

@@ -95,7 +95,7 @@ __device__ __forceinline__ void first_stage(const Column &io, uint32_t N, uint32
     (void)v;
 }
 
-__device__ void column_fft(const Column &io, uint32_t log2n, const cf *tw, const cd *tw64, cf *L, uint32_t j) {
+__device__ void column_fft(const Column &io, uint32_t log2n, const cd *tw64, cf *L, uint32_t j) {
     const uint32_t N = 1u << log2n;
     const uint32_t P = N < 16 ? N : 16;
     const uint32_t r0_log2 = log2n <= 4 ? log2n : (log2n % 4 ? log2n % 4 : 4);
@@ -112,26 +112,20 @@ __device__ void column_fft(const Column &io, uint32_t log2n, const cf *tw, const
     __syncthreads();
     // radix-16 stages: butterfly j reads j + r N/16, twiddles w_{16 NS}^(r (j mod NS))
     const uint32_t stride = N / 16;
-    uint32_t log2ns = r0_log2, off = 0, off64 = 0;
+    uint32_t log2ns = r0_log2, off64 = 0;
     for (;;) {
         const uint32_t NS = 1u << log2ns;
         const bool last = log2ns + 4 == log2n;
         static_for<0, 16>([&](auto r) { v[r] = L[lds_pad(j + decltype(r)::value * stride)]; });
         const uint32_t k = j & (NS - 1);
-        if (log2ns >= (uint32_t)tw64_min_log2ns(1)) {
-            // wide stage: w^(r k) by an FP64 recurrence from the base w^k, each rounded once
-            const cd base = tw64[off64 + k];
-            cd w = base;
-            static_for<1, 16>([&](auto r) {
-                v[r] = mul(v[r], round32(w));
-                if constexpr (decltype(r)::value + 1 < 16) w = mul64(w, base);
-            });
-        } else {
-            static_for<1, 16>([&](auto r) {
-                constexpr int rr = decltype(r)::value;
-                v[rr] = mul(v[rr], tw[off + (rr - 1) * NS + k]);
-            });
-        }
+        // w^(r k) by an FP64 recurrence from the stage's base w^k, each rounded once (FP64-base
+        // level 3, fft_common.hpp: stage_uses_tw64)
+        const cd w1 = tw64[off64 + k];
+        cd w = w1;
+        static_for<1, 16>([&](auto r) {
+            v[r] = mul(v[r], round32(w));
+            if constexpr (decltype(r)::value + 1 < 16) w = mul64(w, w1);
+        });
         dft<16>(v);
         if (last) {
             if (io.live) static_for<0, 16>([&](auto m) {
@@ -143,8 +137,7 @@ __device__ void column_fft(const Column &io, uint32_t log2n, const cf *tw, const
         const uint32_t base = ((j >> log2ns) << (log2ns + 4)) + k;
         static_for<0, 16>([&](auto m) { L[lds_pad(base + decltype(m)::value * NS)] = v[m]; });
         __syncthreads();
-        off += 15 * NS;
-        off64 += NS;  // the FP64 bases of every radix-16 stage are in the table, used or not
+        off64 += NS;
         log2ns += 4;
     }
 }
@@ -163,12 +156,12 @@ __device__ void single_task(const RaggedArgs &a, const RaggedTask &tk, uint32_t 
     io.in_step = io.out_step = 1;
     io.swap_in = io.swap_out = inv;
     io.twiddle = false;
-    column_fft(io, log2n, a.tw32 + tk.tw, a.tw64 + a.size[log2n].tw64, lds + c * lds_span(N), t % T);
+    column_fft(io, log2n, a.tw64 + tk.tw64, lds + c * lds_span(N), t % T);
 }
 
 // one tile (C = 4096 / N_f columns) of one pass of a two-pass item (index scheme of k_pass.hip)
-__device__ void pass_tile(const RaggedArgs &a, const RaggedSize &z, const cf *in, cf *out, uint32_t tw, uint32_t tw64,
-                          bool first, uint32_t tile, bool inv, cf *lds, uint32_t t) {
+__device__ void pass_tile(const RaggedArgs &a, const RaggedSize &z, const cf *in, cf *out, uint32_t tw64, bool first,
+                          uint32_t tile, bool inv, cf *lds, uint32_t t) {
     const uint32_t lf = first ? z.l2 : z.l1;     // this pass's sub-transform
     const uint32_t C = kRaggedTaskPoints >> lf;  // columns per tile
     const uint32_t c = t % C, j = t / C;
@@ -198,7 +191,7 @@ __device__ void pass_tile(const RaggedArgs &a, const RaggedSize &z, const cf *in
         io.swap_out = inv;
         io.low = col;
     }
-    column_fft(io, lf, a.tw32 + tw, a.tw64 + tw64, lds + c * pass_span((int)lf), j);
+    column_fft(io, lf, a.tw64 + tw64, lds + c * pass_span((int)lf), j);
 }
 
 // a run of tiles of one pass of a two-pass item
@@ -207,13 +200,13 @@ __device__ void pass_tiles(const RaggedArgs &a, const RaggedTask &tk, uint32_t l
     const RaggedSize &z = a.size[log2n];
     const uint64_t off = a.item_off[tk.item];
     const uint32_t run = ((tk.word >> 8) & 255u) + 1;
-    // The table offsets go through an empty asm statement each tile, so the compiler treats them
-    // as new values and reloads the twiddles per tile instead of holding every tile's twiddles in
+    // The table offset goes through an empty asm statement each tile, so the compiler treats it
+    // as a new value and reloads the twiddle bases per tile instead of holding every tile's in
     // registers across the loop (which would cost occupancy).
     for (uint32_t i = 0; i < run; i++) {
-        uint32_t tw = first ? z.tw2 : z.tw1, tw64 = first ? z.tw64_2 : z.tw64_1;
-        asm volatile("" : "+v"(tw), "+v"(tw64));
-        pass_tile(a, z, a.in + off, a.out + off, tw, tw64, first, tk.arg + i, inv, lds, t);
+        uint32_t tw64 = first ? z.tw64_2 : z.tw64_1;
+        asm volatile("" : "+v"(tw64));
+        pass_tile(a, z, a.in + off, a.out + off, tw64, first, tk.arg + i, inv, lds, t);
         __syncthreads();  // LDS reuse
     }
 }

@@ -70,8 +70,7 @@ constexpr int pass_wg(int log2nf, int cols) { return cols << (log2nf - pass_log2
 struct PassArgs {
     const cf *in;
     cf *out;
-    const cf *tw;         // stage twiddles of the sub-transform size
-    const cd *tw64;       // FP64 bases of its wide stages (fft_common.hpp: stage_uses_tw64)
+    const cd *tw64;       // FP64 bases of the sub-transform's stages (fft_common.hpp: stage_has_tw64)
     const cd *tw_lo;      // w_n^i, i < 2^lo_bits (FP64)
     const cd *tw_hi;      // w_n^(i 2^lo_bits), i < n / 2^lo_bits (FP64)
     uint64_t in_dist;     // complex values between transforms of a batch
@@ -119,15 +118,16 @@ struct RaggedTask {
     uint32_t word;  // kind | log2n << 2 | inverse << 7 | (tiles - 1) << 8
     uint32_t item;  // single: first index into the sorted item list; others: the item's index
     uint32_t arg;   // single: number of transforms; pass runs: the first tile; whole: 0
-    uint32_t tw;    // single: the stage table of n, as an offset in the FP32 table blob, read per
+    uint32_t tw64;  // single: the FP64 stage bases of n, as an offset in the table blob, read per
                     // task so the compiler cannot hoist twiddle loads out of the task loop
 };
 
+// Every stage after the first of a ragged column transform is radix 16 and takes its twiddles
+// from FP64 bases (fft_common.hpp: stage_uses_tw64, level 3); the offsets below index the blob of
+// FP64 values.
 struct RaggedSize {
-    uint32_t tw;       // single-pass sizes: offset of the stage table in the FP32 table blob
-    uint32_t tw64;     // ... and of its FP64 stage bases in the FP64 blob
+    uint32_t tw64;     // single-pass sizes: offset of the FP64 stage bases
     uint32_t l1, l2;   // two-pass sizes: log2 N_1 (transformed last) and log2 N_2
-    uint32_t tw1, tw2; // stage tables of N_1 and N_2
     uint32_t tw64_1, tw64_2;  // FP64 stage bases of N_1 and N_2
     uint32_t lo, hi;   // offsets of the FP64 twiddle tables of w_n in the FP64 blob
     uint32_t lo_bits;
@@ -139,8 +139,7 @@ struct RaggedArgs {
     const RaggedTask *tasks;
     const uint64_t *item_off;  // complex offset of each item, in task order
     uint32_t *head;            // [0] next task to claim, [1] phase-1 tasks published, [2] error flag
-    const cf *tw32;
-    const cd *tw64;
+    const cd *tw64;            // FP64 stage bases and two-level FP64 tables of w_n
     uint32_t ntasks;
     uint32_t nphase1;          // tasks [0, nphase1) are phase 1
     uint32_t nitems;

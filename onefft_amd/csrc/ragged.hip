@@ -84,24 +84,11 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
     plan->extent = extent;
     plan->items.assign(items, items + count);
 
-    // tables: one FP32 stage table per size used as a single pass or as a sub-transform, and the
-    // two-level FP64 table of w_n per two-pass size
+    // tables, all FP64: the stage bases of every size used as a single pass or as a sub-transform,
+    // and the two-level table of w_n of every two-pass size
     bool used[kRaggedMaxLog2 + 1] = {};
     for (size_t i = 0; i < count; i++) used[items[i].log2n] = true;
-    std::vector<cf> tw32(1);
     std::vector<cd> tw64(1);
-    int32_t stage_at[kRaggedMaxLog2 + 1];
-    for (auto &v : stage_at) v = -1;
-    auto stage_table = [&](int log2) -> uint32_t {
-        if (stage_at[log2] < 0) {
-            const int lp = batch_log2p(log2);
-            const uint32_t entries = stage_tw_size(log2, lp);
-            stage_at[log2] = (int32_t)tw32.size();
-            tw32.resize(tw32.size() + entries + 1);
-            if (entries) fill_stage_table(log2, lp, tw32.data() + stage_at[log2]);
-        }
-        return (uint32_t)stage_at[log2];
-    };
     int32_t stage64_at[kRaggedMaxLog2 + 1];
     for (auto &v : stage64_at) v = -1;
     auto stage64_table = [&](int log2) -> uint32_t {
@@ -120,13 +107,10 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
         if (!used[l]) continue;
         RaggedSize &z = a.size[l];
         if (l <= kRaggedSingleMaxLog2) {
-            z.tw = stage_table(l);
             z.tw64 = stage64_table(l);
         } else {
             z.l1 = (uint32_t)l / 2;
             z.l2 = (uint32_t)l - z.l1;
-            z.tw1 = stage_table((int)z.l1);
-            z.tw2 = stage_table((int)z.l2);
             z.tw64_1 = stage64_table((int)z.l1);
             z.tw64_2 = stage64_table((int)z.l2);
             z.lo_bits = ((uint32_t)l + 1) / 2;
@@ -194,7 +178,7 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
         const uint32_t cap = per_task(it.log2n);
         while (e < count && e - k < cap && items[order[e]].log2n == it.log2n && items[order[e]].inverse == it.inverse)
             e++;
-        tasks.push_back({word(kRaggedSingle, it.log2n, it.inverse), (uint32_t)k, (uint32_t)(e - k), a.size[it.log2n].tw});
+        tasks.push_back({word(kRaggedSingle, it.log2n, it.inverse), (uint32_t)k, (uint32_t)(e - k), a.size[it.log2n].tw64});
         k = e;
     }
     const size_t nphase1 = tasks.size();
@@ -206,18 +190,16 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
         return nullptr;
     }
 
-    // one device allocation: tasks | item offsets | queue head | tw32 | tw64
+    // one device allocation: tasks | item offsets | queue counters | tw64
     auto up = [](size_t v) { return (v + 255) / 256 * 256; };
     const size_t o_tasks = 0;
     const size_t o_off = up(o_tasks + tasks.size() * sizeof(RaggedTask));
     const size_t o_head = up(o_off + item_off.size() * sizeof(uint64_t));
-    const size_t o_tw32 = up(o_head + 4 * sizeof(uint32_t));
-    const size_t o_tw64 = up(o_tw32 + tw32.size() * sizeof(cf));
+    const size_t o_tw64 = up(o_head + 4 * sizeof(uint32_t));
     const size_t bytes = up(o_tw64 + tw64.size() * sizeof(cd));
     std::vector<unsigned char> image(bytes, 0);
     std::memcpy(image.data() + o_tasks, tasks.data(), tasks.size() * sizeof(RaggedTask));
     std::memcpy(image.data() + o_off, item_off.data(), item_off.size() * sizeof(uint64_t));
-    std::memcpy(image.data() + o_tw32, tw32.data(), tw32.size() * sizeof(cf));
     std::memcpy(image.data() + o_tw64, tw64.data(), tw64.size() * sizeof(cd));
     if (hipMalloc(&plan->dmem, bytes) != hipSuccess) {
         (void)hipGetLastError();
@@ -236,7 +218,6 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
     a.tasks = reinterpret_cast<const RaggedTask *>(d + o_tasks);
     a.item_off = reinterpret_cast<const uint64_t *>(d + o_off);
     a.head = reinterpret_cast<uint32_t *>(d + o_head);
-    a.tw32 = reinterpret_cast<const cf *>(d + o_tw32);
     a.tw64 = reinterpret_cast<const cd *>(d + o_tw64);
     a.ntasks = (uint32_t)tasks.size();
     a.nphase1 = (uint32_t)nphase1;

@@ -323,7 +323,7 @@ extern "C" onefft_plan *onefft_plan_create(size_t n) {
         image.resize(off + bytes);
         return off;
     };
-    size_t off_single = 0, off_single64 = 0, off_stage[2][kMaxPasses] = {}, off_stage64[2][kMaxPasses] = {};
+    size_t off_single = 0, off_single64 = 0, off_stage64[2][kMaxPasses] = {};
     size_t off_lo = 0, off_hi = 0;
     if (log2n <= (unsigned)kMaxSinglePassLog2) {
         const int lp = batch_log2p((int)log2n);
@@ -355,8 +355,6 @@ extern "C" onefft_plan *onefft_plan_create(size_t n) {
                 const int f = p - i;  // 1-based factor index
                 const int lf = factors[f - 1];
                 const int lp = pass_log2p(lf);
-                off_stage[r][i] = reserve(stage_tw_size(lf, lp) * sizeof(cf) + sizeof(cf));
-                fill_stage_table(lf, lp, reinterpret_cast<cf *>(image.data() + off_stage[r][i]));
                 off_stage64[r][i] = reserve(stage_tw64_size(lf, lp) * sizeof(cd) + sizeof(cd));
                 fill_stage_table64(lf, lp, reinterpret_cast<cd *>(image.data() + off_stage64[r][i]));
                 PassArgs &a = cfg.pass[i].args;
@@ -404,7 +402,6 @@ extern "C" onefft_plan *onefft_plan_create(size_t n) {
         for (int r = 0; r < 2; r++) {
             for (int i = 0; i < plan->cfg[r].passes; i++) {
                 PassArgs &a = plan->cfg[r].pass[i].args;
-                a.tw = reinterpret_cast<const cf *>(dev(off_stage[r][i]));
                 a.tw64 = reinterpret_cast<const cd *>(dev(off_stage64[r][i]));
                 a.tw_lo = reinterpret_cast<const cd *>(dev(off_lo));
                 a.tw_hi = reinterpret_cast<const cd *>(dev(off_hi));

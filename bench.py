@@ -399,6 +399,42 @@ def report(torch, F, stream, x2, y2, ms2, threads):
     return out
 
 
+def stream_copy(torch, src, dst, stream, warmup=50, reps=200):
+    """The copy kernel of onefft_amd/libonefft_stream.so: the 4096-point kernel's global access
+    pattern (8-byte loads and stores, 32 KiB per workgroup, nontemporal) without its arithmetic,
+    timed like the transform (back to back, HIP events on the launch stream). Its rate is the
+    practical bandwidth ceiling of that pattern; None when the library is not built."""
+    import ctypes
+
+    path = os.path.join(ROOT, "onefft_amd", "libonefft_stream.so")
+    if not os.path.exists(path):
+        return None
+    lib = ctypes.CDLL(path)
+    lib.onefft_stream_copy.restype = ctypes.c_int
+    lib.onefft_stream_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                       ctypes.c_void_p]
+    nbytes = src.numel() * src.element_size()
+    assert dst.numel() * dst.element_size() >= nbytes
+
+    def launch():
+        if lib.onefft_stream_copy(src.data_ptr(), dst.data_ptr(), nbytes, 1, stream.cuda_stream) != 0:
+            raise RuntimeError("onefft_stream_copy failed")
+
+    for _ in range(warmup):
+        launch()
+    e0, e1 = _events(torch)
+    e0.record(stream)
+    for _ in range(reps):
+        launch()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) / reps * 1e3
+    ok = bool(torch.equal(src, dst))
+    return {"kernel": "stream_copy_kernel<true> (onefft_amd/csrc/stream_copy.hip)", "bytes_read": nbytes,
+            "bytes_written": nbytes, "launch_us": round(us, 2), "gb_s": round(2 * nbytes / (us * 1e-6) / 1e9, 1),
+            "copy_exact": ok}
+
+
 def traffic_from_profiles(name: str):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by a
     copy-kernel calibration), committed under profiles/; None when not measured."""
@@ -502,7 +538,9 @@ def main():
         z = torch.empty_like(x)
         z.copy_(x)  # a plain copy of known size: 512 MiB read, 512 MiB written
         torch.cuda.synchronize()
-        print(json.dumps({"kernel_only": True, "ms_per_step": ms, "gb_s": alg_bytes / (ms * 1e-3) / 1e9}), flush=True)
+        sc = stream_copy(torch, x, z, stream)
+        print(json.dumps({"kernel_only": True, "ms_per_step": ms, "gb_s": alg_bytes / (ms * 1e-3) / 1e9,
+                          "stream_copy": sc}), flush=True)
         return
     gflops = flops / (ms * 1e-3) / 1e9
     achieved = alg_bytes / (ms * 1e-3) / 1e9
@@ -534,6 +572,9 @@ def main():
     c1.record(stream)
     torch.cuda.synchronize()
     copy_gbs = alg_bytes / (c0.elapsed_time(c1) / 5 * 1e-3) / 1e9
+    sc = stream_copy(torch, x, z, stream)
+    if sc is not None:
+        sc["fft_frac_of_copy"] = round(achieved / sc["gb_s"], 4)
 
     cpu = {"value": None, "skipped": True} if args.no_cpu else cpu_baseline(args.cpu_seconds)
     if args.report:
@@ -565,6 +606,7 @@ def main():
                      "method": f"{pc} single-transform device calls timed, scaled to {BATCH}",
                      "batched_speedup": round(per_call_ms / ms, 2)},
         "copy_gb_s": round(copy_gbs, 1),
+        "stream_copy": sc,
         "roundtrip_rel_l2": rt,
         "wall_s_timed": round(wall, 4),
         **extras,

@@ -19,6 +19,9 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libonefft_amd.so")
+# measurement only: the bench's copy kernel with the single-pass kernel's access pattern
+STREAM_SOURCE = "stream_copy.hip"
+STREAM_LIB = os.path.join(HERE, "libonefft_stream.so")
 SOURCES = ["runtime.hip", "staging.hip", "ragged.hip", "graph.hip", "tuning.hip", "k_batch.hip", "k_batch_tw.hip", "k_pass.hip",
            "k_ragged.hip"]
 ARCH = "gfx950"
@@ -54,7 +57,8 @@ def build(verbose: bool = True) -> str:
     sources = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     workers = min(len(sources), max(1, min(8, os.cpu_count() or 1)))
     with concurrent.futures.ThreadPoolExecutor(workers) as pool:
-        objs = list(pool.map(_compile, sources))
+        objs = list(pool.map(_compile, sources + [STREAM_SOURCE]))
+    stream_obj = objs.pop()
     script = os.path.join(OBJ, "exports.map")
     with open(script, "w") as f:
         f.write("{ global: onefft_*; local: *; };\n")
@@ -65,8 +69,13 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", stream_obj, "-o", STREAM_LIB + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(STREAM_LIB + ".tmp", STREAM_LIB)
     if verbose:
-        print(f"built {LIB}", flush=True)
+        print(f"built {LIB} and {STREAM_LIB}", flush=True)
     return LIB
 
 

@@ -57,7 +57,9 @@ void onefft_cache_clear(void);
 typedef struct onefft_plan onefft_plan;
 
 /* NULL if n is not a power of two from 1 to 2^30 or if the device allocation fails. The plan
- * belongs to the HIP device that is current when it is created. */
+ * belongs to the HIP device that is current when it is created. For n <= 2^22 it also sets up
+ * the calling thread's stream and staging buffers for host-pointer calls of this size (larger
+ * plans leave that to the first such call on each thread, or to onefft_cache_prepare). */
 onefft_plan *onefft_plan_create(size_t n);
 
 /* Frees a plan; NULL is allowed. */
@@ -86,16 +88,17 @@ const char *onefft_last_error(void);
 
 /* Device-pointer transform, asynchronous on `stream` (reference counterpart:
  * onefft_plan_forward/inverse, include/onefft.h:105-106, which run synchronously on the host).
- * d_in and d_out are device memory, 8-byte aligned, not overlapping; d_in is not modified.
+ * d_in and d_out are device memory, 8-byte aligned, not overlapping (-1 if they do); d_in is
+ * not modified.
  * Returns 0, or -1 (invalid arguments, launch failure; see onefft_last_error()). */
 int onefft_plan_forward_device(const onefft_plan *plan, const float *d_in, float *d_out, struct ihipStream_t *stream);
 int onefft_plan_inverse_device(const onefft_plan *plan, const float *d_in, float *d_out, struct ihipStream_t *stream);
 
 /* `batch` transforms of plan's size in one launch sequence (no reference counterpart; the
  * reference transforms one array per call). Transform b reads d_in + 2*b*in_dist floats and
- * writes d_out + 2*b*out_dist floats; distances are in complex values, at least the size, and the
- * output regions must not overlap each other or the input. Asynchronous on `stream`. Returns 0
- * or -1. */
+ * writes d_out + 2*b*out_dist floats; distances are in complex values, at least the size, so the
+ * output regions do not overlap each other. The input and output ranges must not overlap: -1 if
+ * they do. Asynchronous on `stream`. Returns 0 or -1. */
 int onefft_plan_forward_batch(const onefft_plan *plan, const float *d_in, float *d_out, size_t batch,
                               size_t in_dist, size_t out_dist, struct ihipStream_t *stream);
 int onefft_plan_inverse_batch(const onefft_plan *plan, const float *d_in, float *d_out, size_t batch,
@@ -126,9 +129,10 @@ void onefft_ragged_plan_destroy(onefft_ragged_plan *plan);
 size_t onefft_ragged_extent(const onefft_ragged_plan *plan);
 size_t onefft_ragged_task_count(const onefft_ragged_plan *plan);
 
-/* Transforms every item in one persistent-kernel launch, asynchronously on `stream`. d_in is not
- * modified. A ragged plan runs one execution at a time: executions of one plan must be ordered
- * (same stream or synchronised). Returns 0 or -1. */
+/* Transforms every item in one persistent-kernel launch, asynchronously on `stream`. d_in and
+ * d_out are device memory, 8-byte aligned, each holding onefft_ragged_extent() complex values,
+ * not overlapping (-1 if they do); d_in is not modified. A ragged plan runs one execution at a
+ * time: executions of one plan must be ordered (same stream or synchronised). Returns 0 or -1. */
 int onefft_ragged_execute(const onefft_ragged_plan *plan, const float *d_in, float *d_out, struct ihipStream_t *stream);
 
 /* The same result with one device plan call per item, for comparison. */

@@ -19,15 +19,18 @@ struct onefft_graph {
 
 namespace {
 
+// Records `record`'s launches on a capture stream of `device` (the plan's), which stays current
+// while the graph is built.
 template <typename Record>
-onefft_graph *capture(Record record) {
-    onefft_graph *g = new (std::nothrow) onefft_graph;
-    if (!g) return nullptr;
-    if (hipGetDevice(&g->device) != hipSuccess) {
-        onefft::set_error("no HIP device");
-        delete g;
+onefft_graph *capture(int device, Record record) {
+    onefft::DeviceGuard guard(device);
+    if (!guard.ok) {
+        onefft::set_error("cannot select the plan's device");
         return nullptr;
     }
+    onefft_graph *g = new (std::nothrow) onefft_graph;
+    if (!g) return nullptr;
+    g->device = device;
     hipStream_t s = nullptr;
     hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -63,16 +66,8 @@ onefft_graph *capture(Record record) {
 
 extern "C" onefft_graph *onefft_graph_create_batch(const onefft_plan *plan, const float *d_in, float *d_out,
                                                    size_t batch, size_t in_dist, size_t out_dist, int inverse) {
-    if (!plan || !d_in || !d_out || (((uintptr_t)d_in | (uintptr_t)d_out) & 7)) {
-        onefft::set_error("null plan, or device buffers not 8-byte aligned");
-        return nullptr;
-    }
-    const size_t n = onefft_plan_size(plan);
-    if (batch > 1 && (in_dist < n || out_dist < n)) {
-        onefft::set_error("batch distance smaller than the transform size");
-        return nullptr;
-    }
-    return capture([&](hipStream_t s) {
+    if (!onefft::device_args_ok(plan, d_in, d_out, batch, in_dist, out_dist)) return nullptr;
+    return capture(onefft::plan_device(plan), [&](hipStream_t s) {
         return onefft::execute_device(plan, reinterpret_cast<const onefft::cf *>(d_in), reinterpret_cast<onefft::cf *>(d_out),
                                       batch, in_dist, out_dist, inverse != 0, s);
     });
@@ -83,12 +78,18 @@ extern "C" onefft_graph *onefft_graph_create_ragged(const onefft_ragged_plan *pl
         onefft::set_error("null plan");
         return nullptr;
     }
-    return capture([&](hipStream_t s) { return onefft_ragged_execute(plan, d_in, d_out, s); });
+    return capture(onefft::ragged_plan_device(plan),
+                   [&](hipStream_t s) { return onefft_ragged_execute(plan, d_in, d_out, s); });
 }
 
 extern "C" int onefft_graph_launch(const onefft_graph *graph, struct ihipStream_t *stream) {
     if (!graph) {
         onefft::set_error("null graph");
+        return -1;
+    }
+    onefft::DeviceGuard guard(graph->device);
+    if (!guard.ok) {
+        onefft::set_error("cannot select the graph's device");
         return -1;
     }
     hipError_t e = hipGraphLaunch(graph->exec, stream);
@@ -101,6 +102,7 @@ extern "C" int onefft_graph_launch(const onefft_graph *graph, struct ihipStream_
 
 extern "C" void onefft_graph_destroy(onefft_graph *graph) {
     if (!graph) return;
+    onefft::DeviceGuard guard(graph->device);
     if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
     if (graph->graph) (void)hipGraphDestroy(graph->graph);
     delete graph;

@@ -239,7 +239,10 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
 
 extern "C" void onefft_ragged_plan_destroy(onefft_ragged_plan *plan) {
     if (!plan) return;
-    if (plan->dmem) (void)hipFree(plan->dmem);
+    if (plan->dmem) {
+        DeviceGuard g(plan->device);
+        (void)hipFree(plan->dmem);
+    }
     delete plan;
 }
 
@@ -247,6 +250,10 @@ extern "C" size_t onefft_ragged_extent(const onefft_ragged_plan *plan) { return 
 
 extern "C" size_t onefft_ragged_task_count(const onefft_ragged_plan *plan) { return plan ? plan->args.ntasks : 0; }
 
+int onefft::ragged_plan_device(const onefft_ragged_plan *plan) { return plan->device; }
+
+// Both buffers hold `extent` complex values; the two-pass items' first passes read the input
+// while other workgroups write the output, so the two must not overlap.
 static bool ragged_args_ok(const onefft_ragged_plan *plan, const float *d_in, float *d_out) {
     if (!plan || (plan->count && (!d_in || !d_out))) {
         set_error("null plan or buffer");
@@ -256,12 +263,22 @@ static bool ragged_args_ok(const onefft_ragged_plan *plan, const float *d_in, fl
         set_error("device buffers must be 8-byte aligned");
         return false;
     }
+    const unsigned __int128 bytes = (unsigned __int128)plan->extent * sizeof(cf);
+    if (!disjoint(d_in, bytes, d_out, bytes)) {
+        set_error("input and output overlap");
+        return false;
+    }
     return true;
 }
 
 extern "C" int onefft_ragged_execute(const onefft_ragged_plan *plan, const float *d_in, float *d_out,
                                      struct ihipStream_t *stream) {
     if (!ragged_args_ok(plan, d_in, d_out)) return -1;
+    DeviceGuard g(plan->device);
+    if (!g.ok) {
+        set_error("cannot select the plan's device");
+        return -1;
+    }
     RaggedArgs a = plan->args;
     a.in = reinterpret_cast<const cf *>(d_in);
     a.out = reinterpret_cast<cf *>(d_out);
@@ -276,6 +293,11 @@ extern "C" int onefft_ragged_execute(const onefft_ragged_plan *plan, const float
 extern "C" int onefft_ragged_execute_per_item(const onefft_ragged_plan *plan, const float *d_in, float *d_out,
                                               struct ihipStream_t *stream) {
     if (!ragged_args_ok(plan, d_in, d_out)) return -1;
+    DeviceGuard g(plan->device);
+    if (!g.ok) {
+        set_error("cannot select the plan's device");
+        return -1;
+    }
     const cf *in = reinterpret_cast<const cf *>(d_in);
     cf *out = reinterpret_cast<cf *>(d_out);
     for (const onefft_ragged_item &it : plan->items) {
@@ -290,6 +312,11 @@ extern "C" int onefft_ragged_execute_per_item(const onefft_ragged_plan *plan, co
 extern "C" int onefft_ragged_wait(const onefft_ragged_plan *plan, struct ihipStream_t *stream) {
     if (!plan) {
         set_error("null plan");
+        return -1;
+    }
+    DeviceGuard g(plan->device);
+    if (!g.ok) {
+        set_error("cannot select the plan's device");
         return -1;
     }
     hipError_t e = hipStreamSynchronize(stream);

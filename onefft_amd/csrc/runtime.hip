@@ -26,6 +26,10 @@ using onefft::cf;
 
 namespace {
 constexpr unsigned kMaxLog2 = 30;  // the reference's limit (src/onefft.c:1117-1121)
+// Plans up to this size also set up the creating thread's host-call staging (2 x 32 MiB of device
+// memory at most), so that thread's host-pointer calls allocate nothing; larger plans, often
+// used with device pointers only, leave it to the first host-pointer call or onefft_cache_prepare.
+constexpr unsigned kPlanStagingMaxLog2 = 22;
 }  // namespace
 
 namespace onefft {
@@ -170,10 +174,9 @@ int choose_split_factors(unsigned log2n, int regime, int *factors) {
     return p;
 }
 
-// Single-pass kernel variant: the tuning table's choice, else nontemporal global access with the
-// padded LDS layout (measured fastest on MI355X for 2^8..2^14 points: the swizzled layout's extra
-// occupancy does not pay). ONEFFT_BATCH_VARIANT="nt,layout" (0/1 each) overrides both, for
-// measurement.
+// Single-pass kernel variant: the tuning table's choice, else the BatchVariant defaults
+// (kernels.hpp). ONEFFT_BATCH_VARIANT="nt,layout[,tw64]" (nt and layout 0/1, tw64 the FP64-base
+// level 0..3, 1 when omitted) overrides the table, for measurement.
 onefft::BatchVariant choose_variant(unsigned log2n, int regime) {
     onefft::BatchVariant v;
     const onefft::TuneEntry &t = onefft::tune_lookup(log2n, regime);
@@ -237,29 +240,21 @@ bool size_log2(size_t n, unsigned *log2n) {
     return true;
 }
 
-namespace {
-
-// RAII: make the plan's device current for the duration of a call.
-struct DeviceGuard {
-    int previous = -1;
-    bool ok = true;
-    explicit DeviceGuard(int device) {
-        int cur = 0;
-        if (hipGetDevice(&cur) != hipSuccess) {
-            ok = false;
-            return;
-        }
-        if (cur != device) {
-            if (hipSetDevice(device) != hipSuccess) ok = false;
-            else previous = cur;
-        }
+DeviceGuard::DeviceGuard(int device) {
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) {
+        ok = false;
+        return;
     }
-    ~DeviceGuard() {
-        if (previous >= 0) (void)hipSetDevice(previous);
+    if (cur != device) {
+        if (hipSetDevice(device) != hipSuccess) ok = false;
+        else previous = cur;
     }
-};
+}
 
-}  // namespace
+DeviceGuard::~DeviceGuard() {
+    if (previous >= 0) (void)hipSetDevice(previous);
+}
 
 // Launches the transform of `batch` arrays on `stream`; data pointers are device memory.
 int execute_device(const onefft_plan *plan, const cf *d_in, cf *d_out, uint64_t batch, uint64_t in_dist,
@@ -408,7 +403,7 @@ extern "C" onefft_plan *onefft_plan_create(size_t n) {
             }
         }
     }
-    staging_reserve(plan->device, n);
+    if (log2n <= kPlanStagingMaxLog2) (void)staging_reserve(plan->device, n);
     return plan;
 }
 
@@ -447,20 +442,35 @@ extern "C" void onefft_plan_inverse(const onefft_plan *plan, const float *in, fl
     (void)run_sync(plan, in, out, true);
 }
 
-static int run_device(const onefft_plan *plan, const float *d_in, float *d_out, size_t batch, size_t in_dist,
-                      size_t out_dist, bool inverse, hipStream_t stream) {
+int onefft::plan_device(const onefft_plan *plan) { return plan->device; }
+
+bool onefft::device_args_ok(const onefft_plan *plan, const void *d_in, const void *d_out, size_t batch,
+                            size_t in_dist, size_t out_dist) {
     if (!plan || !d_in || !d_out) {
         set_error("null plan or buffer");
-        return -1;
+        return false;
     }
     if (((uintptr_t)d_in | (uintptr_t)d_out) & 7) {
         set_error("device buffers must be 8-byte aligned");
-        return -1;
+        return false;
     }
     if (batch > 1 && (in_dist < plan->n || out_dist < plan->n)) {
         set_error("batch distance smaller than the transform size");
-        return -1;
+        return false;
     }
+    if (batch == 0) return true;
+    // bytes from the first transform's start to the last one's end
+    const auto span = [&](size_t dist) { return ((unsigned __int128)(batch - 1) * dist + plan->n) * sizeof(cf); };
+    if (!disjoint(d_in, span(in_dist), d_out, span(out_dist))) {
+        set_error("input and output overlap");
+        return false;
+    }
+    return true;
+}
+
+static int run_device(const onefft_plan *plan, const float *d_in, float *d_out, size_t batch, size_t in_dist,
+                      size_t out_dist, bool inverse, hipStream_t stream) {
+    if (!device_args_ok(plan, d_in, d_out, batch, in_dist, out_dist)) return -1;
     DeviceGuard g(plan->device);
     if (!g.ok) {
         set_error("cannot select the plan's device");

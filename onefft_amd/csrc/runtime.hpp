@@ -9,6 +9,7 @@
 #include "kernels.hpp"
 
 struct onefft_plan;
+struct onefft_ragged_plan;
 
 namespace onefft {
 
@@ -57,5 +58,33 @@ const TuneEntry &tune_lookup(unsigned log2n, int regime);
 
 // The calling thread's stream on `device` (created on first use), or nullptr on failure.
 hipStream_t thread_stream(int device);
+
+// Makes `device` current for the guard's lifetime and restores the previous device after; `ok`
+// is false if the device could not be selected.
+struct DeviceGuard {
+    int previous = -1;
+    bool ok = true;
+    explicit DeviceGuard(int device);
+    ~DeviceGuard();
+    DeviceGuard(const DeviceGuard &) = delete;
+    DeviceGuard &operator=(const DeviceGuard &) = delete;
+};
+
+// The device a plan or a ragged plan belongs to.
+int plan_device(const onefft_plan *plan);
+int ragged_plan_device(const onefft_ragged_plan *plan);
+
+// Checks the device-pointer arguments of a batched call: non-null, 8-byte aligned, distances of
+// at least the size when batch > 1, and input and output byte ranges that do not overlap (the
+// passes of a multi-pass transform read input that other workgroups would already have
+// overwritten). Records the reason and returns false otherwise.
+bool device_args_ok(const onefft_plan *plan, const void *d_in, const void *d_out, size_t batch, size_t in_dist,
+                    size_t out_dist);
+
+// True if the byte ranges [a, a + alen) and [b, b + blen) are disjoint.
+inline bool disjoint(const void *a, unsigned __int128 alen, const void *b, unsigned __int128 blen) {
+    const unsigned __int128 x = (uintptr_t)a, y = (uintptr_t)b;
+    return alen == 0 || blen == 0 || x + alen <= y || y + blen <= x;
+}
 
 }  // namespace onefft

@@ -237,7 +237,9 @@ int execute_host_call(const onefft_plan *plan, const float *in, float *out, bool
         return -1;
     }
     const Mem in_kind = memory_kind(in), out_kind = memory_kind(out);
-    const bool in_direct = in_kind == Mem::kDevice && ((uintptr_t)in & 7) == 0;
+    // device buffers are used where they lie when aligned; an input that overlaps the output
+    // (which the API does not allow) is still copied first, so the result stays correct
+    const bool in_direct = in_kind == Mem::kDevice && ((uintptr_t)in & 7) == 0 && disjoint(in, bytes, out, bytes);
     const bool out_direct = out_kind == Mem::kDevice && ((uintptr_t)out & 7) == 0;
     if (!staging_reserve(device, (in_direct && out_direct) ? 0 : n)) return -1;
     Staging &s = t_stage;

@@ -180,6 +180,48 @@ def test_device_pointers_misaligned_and_mixed():
         assert O.rel_rms(got, want) < tol_ref(12)
 
 
+@pytest.mark.parametrize("L", [12, 16])
+def test_host_call_with_aliased_device_buffers(L):
+    """The drop-in call gets the same device buffer as input and output (which the API does not
+    allow, reference include/onefft.h:61-64): the input is staged first, so the result is the
+    out-of-place one, bit for bit, for a single-pass and a multi-pass size."""
+    n = 1 << L
+    print(f"aliased host call n=2^{L}", flush=True)
+    p = F.Plan(n)
+    x = O.uniform(40 + L, n)
+    want = torch.empty(2 * n, device="cuda")
+    d = torch.from_numpy(x).cuda()
+    p.forward_ptr(d.data_ptr(), want.data_ptr())
+    p.forward_ptr(d.data_ptr(), d.data_ptr())
+    torch.cuda.synchronize()
+    assert torch.equal(d, want)
+
+
+def test_device_calls_reject_overlap():
+    """Device-pointer batch calls, ragged executions and graph captures whose input and output
+    ranges overlap return -1 (NULL for a graph) with a message, and launch nothing."""
+    L = F.lib()
+    n = 1 << 16
+    p = F.Plan(n)
+    buf = torch.zeros(2 * (4 * n), device="cuda")
+    a = buf.data_ptr()
+    sp = stream()
+    for d_in, d_out, batch in ((a, a, 1), (a, a + 8 * (n - 1), 1), (a + 8 * n, a, 2), (a, a + 8 * (2 * n), 3)):
+        assert L.onefft_plan_forward_batch(p.handle, d_in, d_out, batch, n, n, sp) == -1
+        assert "overlap" in F.last_error()
+        assert not L.onefft_graph_create_batch(p.handle, d_in, d_out, batch, n, n, 0)
+    # adjacent, not overlapping: accepted
+    assert L.onefft_plan_forward_batch(p.handle, a, a + 8 * n, 1, n, n, sp) == 0
+    assert L.onefft_plan_forward_batch(p.handle, a, a + 8 * (2 * n), 2, n, n, sp) == 0
+    r = F.RaggedPlan([(0, 10, 0), (1024, 16, 1)])
+    ext = r.extent
+    assert L.onefft_ragged_execute(r._p, a, a + 8 * (ext - 1), sp) == -1
+    assert "overlap" in F.last_error()
+    assert L.onefft_ragged_execute(r._p, a, a + 8 * ext, sp) == 0
+    r.wait(sp)
+    torch.cuda.synchronize()
+
+
 # ---- batches ---------------------------------------------------------------------------------------
 
 @pytest.mark.parametrize("L,batch", [(0, 5), (1, 3), (3, 257), (4, 1000), (5, 129), (8, 77), (10, 33), (11, 5),

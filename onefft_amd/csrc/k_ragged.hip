@@ -3,10 +3,10 @@
 //
 // The reference transforms one array per call (include/onefft.h:61-77); a caller with many
 // arrays of mixed sizes calls it once per array. Here every array of a batch is an item
-// (offset, log2 n, direction). A task is up to 4096 / n whole single-pass items of one size
-// n <= 2^12, one whole two-pass item below the split size (both passes by one workgroup, whose
-// barriers order them), or a run of tiles (C = 4096 / N_f columns each) of one pass of a larger
-// two-pass item, so a large item is spread over many workgroups (kernels.hpp).
+// (offset, log2 n, direction). A task is up to S / n whole single-pass items of one size n <= S
+// (S = kRaggedTaskPoints), one whole two-pass item below the split size (both passes by one
+// workgroup, whose barriers order them), or a run of tiles (C = S / N_f columns each) of one pass
+// of a larger two-pass item, so a large item is spread over many workgroups (kernels.hpp).
 //
 // The queue runs in two phases: single-pass, whole and first-pass tasks, then second-pass runs,
 // which read what the first passes wrote. Device-scope release and acquire on this chip write
@@ -24,8 +24,8 @@ namespace onefft {
 
 namespace {
 
-// LDS of one task: the largest of the single-pass layouts (4096 points) and the pass layouts
-// (C = 4096 / N_f columns of pass_span(N_f) values).
+// LDS of one task: the largest of the single-pass layouts (kRaggedTaskPoints points) and the pass
+// layouts (C = kRaggedTaskPoints / N_f columns of pass_span(N_f) values).
 constexpr uint32_t ragged_lds_values() {
     uint32_t m = kRaggedTaskPoints / 16 * lds_span(16);
     for (int l = 6; l <= 10; l++) {
@@ -36,7 +36,8 @@ constexpr uint32_t ragged_lds_values() {
 }
 constexpr uint32_t kLds = ragged_lds_values();
 
-// One column transform of N = 2^L points (L <= 12) with every stage's shape chosen at run time:
+// One column transform of N = 2^L points (L <= kRaggedTaskLog2) with every stage's shape chosen
+// at run time:
 // a first stage of radix R0 (N itself below 16 points, else 2^(L mod 4), or 16), then radix-16
 // stages through LDS. The kernel thus has a few short code paths instead of one fully unrolled
 // body per size, which keeps its register use near that of the standalone kernels.
@@ -142,7 +143,8 @@ __device__ void column_fft(const Column &io, uint32_t log2n, const cd *tw64, cf 
     }
 }
 
-// up to 4096 / n whole transforms of one size n <= 2^12, one column per transform
+// up to kRaggedTaskPoints / n whole transforms of one size n <= kRaggedTaskPoints, one column per
+// transform
 __device__ void single_task(const RaggedArgs &a, const RaggedTask &tk, uint32_t log2n, bool inv, cf *lds,
                             uint32_t t) {
     const uint32_t N = 1u << log2n;
@@ -159,7 +161,8 @@ __device__ void single_task(const RaggedArgs &a, const RaggedTask &tk, uint32_t 
     column_fft(io, log2n, a.tw64 + tk.tw64, lds + c * lds_span(N), t % T);
 }
 
-// one tile (C = 4096 / N_f columns) of one pass of a two-pass item (index scheme of k_pass.hip)
+// one tile (C = kRaggedTaskPoints / N_f columns) of one pass of a two-pass item (index scheme of
+// k_pass.hip)
 __device__ void pass_tile(const RaggedArgs &a, const RaggedSize &z, const cf *in, cf *out, uint32_t tw64, bool first,
                           uint32_t tile, bool inv, cf *lds, uint32_t t) {
     const uint32_t lf = first ? z.l2 : z.l1;     // this pass's sub-transform
@@ -242,9 +245,12 @@ __global__ __launch_bounds__(kRaggedWG) void fft_ragged_kernel(RaggedArgs a) {
                         }
                         __builtin_amdgcn_s_sleep(8);
                     }
+                    // the thread that read the count acquires the publishers' releases ...
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                 }
                 __syncthreads();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // every wave drops stale cached lines
+                // ... and, after the barrier, every wave drops its stale cached lines
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             }
         }
         if (id >= a.ntasks) break;

@@ -94,11 +94,11 @@ hipError_t launch_pass(int log2nf, int cols, bool first, bool swap_in, bool swap
 
 // ---- ragged batches: one persistent launch over items of mixed sizes -------------------------
 //
-// A task is one of
-//   * up to 4096 / n whole transforms of one size n <= 2^12 (single-pass);
-//   * one whole two-pass transform of 2^13 points up to below the split size (both passes, tile
-//     by tile: a tile is 4096 points, C = 4096 / N_f columns of an N_f-point sub-transform), run
-//     by one workgroup, whose second pass then reads its first pass's output from cache;
+// A task covers up to S = kRaggedTaskPoints points (16 per thread of a workgroup) and is one of
+//   * up to S / n whole transforms of one size n <= S (single-pass);
+//   * one whole two-pass transform above S and below the split size (both passes, tile by tile:
+//     a tile is S points, C = S / N_f columns of an N_f-point sub-transform), run by one
+//     workgroup, whose second pass then reads its first pass's output from cache;
 //   * a run of up to kRaggedTileRun tiles of the first pass of a two-pass transform of the split
 //     size or larger (up to 2^20 points), so one large transform is spread over many workgroups;
 //   * a run of tiles of such a transform's second pass.
@@ -106,11 +106,13 @@ hipError_t launch_pass(int log2nf, int cols, bool first, bool swap_in, bool swap
 // runs (phase 2); a workgroup starts its first phase-2 task only once every phase-1 task's
 // results are published device-wide.
 constexpr int kRaggedMaxLog2 = 20;
-constexpr int kRaggedSingleMaxLog2 = 12;
-constexpr int kRaggedWG = 256;
-constexpr uint32_t kRaggedTaskPoints = 4096;
-constexpr uint32_t kRaggedTileRun = 16;  // tiles per pass task (at most 256)
-constexpr int kRaggedSplitLog2 = 19;     // default split size (ONEFFT_RAGGED_SPLIT overrides)
+constexpr int kRaggedTaskLog2 = 13;     // 512-thread workgroups; 8192-point tiles give the 2^20
+                                        // items' passes runs of 8 values (4096: 4, 1.35x slower)
+constexpr uint32_t kRaggedTaskPoints = 1u << kRaggedTaskLog2;
+constexpr int kRaggedSingleMaxLog2 = kRaggedTaskLog2;
+constexpr int kRaggedWG = kRaggedTaskPoints / 16;
+constexpr uint32_t kRaggedTileRun = 2;   // tiles per pass task (at most 256)
+constexpr int kRaggedSplitLog2 = 20;     // default split size (ONEFFT_RAGGED_SPLIT overrides)
 
 enum RaggedKind : uint32_t { kRaggedSingle = 0, kRaggedFirst = 1, kRaggedSecond = 2, kRaggedWhole = 3 };
 

@@ -157,7 +157,7 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
         for (size_t k = 0; k < nbig; k++) {
             const onefft_ragged_item &it = items[order[k]];
             if (it.log2n < split) continue;
-            const uint32_t tiles = 1u << (it.log2n - 12);  // 4096-point tiles per pass
+            const uint32_t tiles = 1u << (it.log2n - kRaggedTaskLog2);  // tiles per pass
             for (uint32_t t0 = 0; t0 < tiles; t0 += kRaggedTileRun) {
                 const uint32_t run = tiles - t0 < kRaggedTileRun ? tiles - t0 : kRaggedTileRun;
                 tasks.push_back({word(kind, it.log2n, it.inverse, run), (uint32_t)k, t0, 0});
@@ -169,7 +169,8 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
     for (size_t k = 0; k < nbig; k++) {
         const onefft_ragged_item &it = items[order[k]];
         if (it.log2n < split)
-            tasks.push_back({word(kRaggedWhole, it.log2n, it.inverse, 1u << (it.log2n - 12)), (uint32_t)k, 0, 0});
+            tasks.push_back(
+                {word(kRaggedWhole, it.log2n, it.inverse, 1u << (it.log2n - kRaggedTaskLog2)), (uint32_t)k, 0, 0});
     }
     pass_tasks(kRaggedFirst);
     for (size_t k = nbig; k < count;) {

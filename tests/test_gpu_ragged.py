@@ -70,7 +70,7 @@ def test_every_size_with_gaps(per_item):
 
 def test_many_small_items_fill_tasks():
     rng = np.random.default_rng(5)
-    sizes = rng.integers(0, 13, 3000)
+    sizes = rng.integers(0, 14, 3000)
     items, extent = layout(sizes, seed=2)
     x = O.uniform(32, extent)
     y, p = run(items, extent, x)
@@ -126,7 +126,7 @@ def test_phase_wait_timeout_is_reported(monkeypatch):
     """Failure handling, forced from the host: a plan whose bounded wait for the first phase gives
     up at once (limit 0) still runs the launch to its end (every access stays inside the items),
     and onefft_ragged_wait reports the timeout; a default plan of the same items is then correct."""
-    items, extent = layout([19] + [10] * 200, seed=5)  # the 2^19 item's second passes wait
+    items, extent = layout([20] + [10] * 200, seed=5)  # the 2^20 item's second passes wait
     x = O.normal(33, extent)
     d_in = torch.from_numpy(x).cuda()
     d_out = torch.zeros(2 * extent, device="cuda")

@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--single-transform", type=int, default=None, metavar="LOG2N",
                     help="instead of the benchmark: 3 forward transforms of one 2^LOG2N-point input (random, "
                          "resident in HBM) and one device copy of the same bytes, for counter runs")
+    ap.add_argument("--ragged-only", action="store_true",
+                    help="instead of the benchmark: 3 one-launch executions of the 4 GiB ragged prefix (seed-3 items, "
+                         "random input resident in HBM), for counter runs")
     ap.add_argument("--report", action="store_true",
                     help="add a per-configuration report: errors against the reference CPU transform and the "
                          "FP64 oracle, and the reference's CPU times (several minutes of host work)")
@@ -498,6 +501,19 @@ def main():
         z.copy_(x)  # a plain copy of known size: 8 n bytes read, 8 n bytes written
         torch.cuda.synchronize()
         print(json.dumps({"single_transform": args.single_transform, "launches": 3}), flush=True)
+        return
+    if args.ragged_only:
+        items = F.ragged_items(3, 200000, max_bytes=4 << 30)
+        extent = items[-1][0] + (1 << items[-1][1])
+        x = torch.randn(extent, dtype=torch.complex64, device="cuda")
+        y = torch.empty_like(x)
+        rp = F.RaggedPlan(items)
+        sp = torch.cuda.current_stream().cuda_stream
+        for _ in range(3):
+            rp.execute(x.data_ptr(), y.data_ptr(), sp)
+        rp.wait(sp)
+        print(json.dumps({"ragged_only": True, "items": len(items), "extent": extent, "launches": 3,
+                          "algorithmic_bytes": 16 * extent}), flush=True)
         return
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)

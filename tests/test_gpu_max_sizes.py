@@ -60,3 +60,34 @@ def test_tone_roundtrip_parseval(L):
     p.close()
     del x, y, z
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("inverse", [False, True])
+def test_multipass_batch_across_launch_chunks(inverse):
+    """A multi-pass batch larger than one launch's grid holds (65535 transforms: the batch index
+    is the grid's y dimension) runs in chunks; the transforms on both sides of the chunk boundary,
+    with input and output distances that differ from the size and from each other, match FP64.
+    65537 transforms of 2^15 points: 16 GiB in and 16 GiB out."""
+    import numpy as np
+
+    L, batch = 15, 65537
+    n = 1 << L
+    in_dist, out_dist = n + 8, n + 24
+    print(f"chunked batch n=2^{L} x{batch} inverse={inverse}", flush=True)
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(7)
+    x = torch.randn(batch * in_dist, dtype=torch.complex64, device=dev, generator=g)
+    y = torch.zeros(batch * out_dist, dtype=torch.complex64, device=dev)
+    F.plan(n).batch_ptr(x.data_ptr(), y.data_ptr(), batch, in_dist, out_dist, inverse, stream())
+    torch.cuda.synchronize()
+    for b in (0, 1, 65534, 65535, 65536):
+        xs = x[b * in_dist: b * in_dist + n].cpu().numpy().astype(np.complex128)
+        want = np.fft.ifft(xs) * n if inverse else np.fft.fft(xs)
+        got = y[b * out_dist: b * out_dist + n].cpu().numpy().astype(np.complex128)
+        err = np.linalg.norm(got - want) / np.linalg.norm(want)
+        assert err < tol_fp64(L), (b, err)
+        # the gap after each output is untouched
+        assert bool((y[b * out_dist + n: (b + 1) * out_dist] == 0).all()), b
+    del x, y
+    torch.cuda.empty_cache()

@@ -9,9 +9,11 @@ tuning table the library reads at plan creation (tuning.hip).
 * larger sizes: every split n = N_1 ... N_p into 2..4 passes with factors 2^6..2^10 (4 passes only
   from 2^24), each with the default and the widest allowed columns per workgroup;
 each in two working-set regimes, because the best choice differs between them: a batch whose
-input plus output fit the 256 MiB Infinity Cache (16 n batch = 256 MiB, or one transform), and a
-batch of 1 GiB in and 1 GiB out (one transform above 2^26 points). The library picks the regime
-per call from n and the batch (runtime.hpp: tune_regime).
+input plus output fit the 256 MiB Infinity Cache (16 n batch = 256 MiB up to 2^20 points; one
+transform from 2^21 points, where a single large transform is the common use and has too few tiles
+per pass for the batch's best split), and a batch of 1 GiB in and 1 GiB out (one transform above
+2^26 points). The library picks the regime per call from n and the batch (runtime.hpp:
+tune_regime).
 
 Each candidate is a plan created under the environment overrides (ONEFFT_BATCH_VARIANT,
 ONEFFT_PASS_SPLIT, ONEFFT_PASS_COLS) and timed with HIP events: the median of 5 launches after 2
@@ -169,7 +171,7 @@ def main(argv=None):
             if regime == 0:
                 if 16 * n > (256 << 20):
                     continue
-                batch = max(1, (1 << 24) // n)
+                batch = 1 if L >= 21 else max(1, (1 << 24) // n)
             else:
                 batch = max(1, (1 << 26 if single else 1 << 27) // n)
                 if 16 * n * batch <= (256 << 20):

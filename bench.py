@@ -452,6 +452,30 @@ def traffic_from_profiles(name: str):
         return json.load(f).get("bytes_per_launch")
 
 
+def host_cpu() -> dict:
+    """The host CPU as /proc/cpuinfo describes it: model, sockets, logical CPUs, physical cores."""
+    model, sockets, cores, logical = None, set(), set(), 0
+    phys = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                key, _, val = line.partition(":")
+                key, val = key.strip(), val.strip()
+                if key == "processor":
+                    logical += 1
+                elif key == "model name" and model is None:
+                    model = val
+                elif key == "physical id":
+                    phys = val
+                    sockets.add(val)
+                elif key == "core id":
+                    cores.add((phys, val))
+    except OSError:
+        pass
+    return {"model": model, "sockets": len(sockets) or None, "logical_cpus": logical or None,
+            "physical_cores": len(cores) or None}
+
+
 def cpu_baseline(budget_s: float):
     """The reference's own transform (oracle/_ref, built from its sources) on this host's cores,
     on a bounded sample of the same workload: a slice of the batch, all threads sharing one plan."""
@@ -477,6 +501,7 @@ def cpu_baseline(budget_s: float):
         "sample": f"{sample} of the {BATCH} transforms of n={N} (re/im N(0,1), seed 1), {threads} threads "
                   f"sharing one plan, min over {samples} samples (bench/bench.c method)",
         "gb_s": round(16.0 * N * sample / per / 1e9, 2),
+        "host": host_cpu(),
     }
 
 
@@ -565,6 +590,17 @@ def main():
     gflops = flops / (ms * 1e-3) / 1e9
     achieved = alg_bytes / (ms * 1e-3) / 1e9
 
+    # per-launch distribution: 50 more launches, each between its own pair of HIP events
+    per_launch = []
+    for _ in range(50):
+        e0, e1 = _events(torch)
+        e0.record(stream)
+        step()
+        e1.record(stream)
+        per_launch.append((e0, e1))
+    torch.cuda.synchronize()
+    per_launch_us = sorted(a.elapsed_time(b) * 1e3 for a, b in per_launch)
+
     # round trip on the device: inverse(forward(x)) = n x
     z = torch.empty_like(x)
     plan.batch_ptr(y.data_ptr(), z.data_ptr(), BATCH, N, N, True, sp)
@@ -620,6 +656,9 @@ def main():
                    "n": N, "batch": BATCH, "bytes_in": 8 * N * BATCH, "bytes_out": 8 * N * BATCH},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles("batch4096")},
+        "per_launch_us": {"launches": len(per_launch_us), "min": round(per_launch_us[0], 2),
+                          "median": round(per_launch_us[len(per_launch_us) // 2], 2),
+                          "note": "50 launches after the timed steps, each timed alone; value uses the timed steps"},
         "cpu_baseline": cpu,
         "per_call": {"value": round(flops / (per_call_ms * 1e-3) / 1e9, 1), "unit": "GFLOP/s",
                      "ms_for_batch": round(per_call_ms, 3),

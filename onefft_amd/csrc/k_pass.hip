@@ -60,6 +60,8 @@ template <int LOG2NF>
 static hipError_t launch_pass_n(int cols, bool first, bool swap_in, bool swap_out, const PassArgs &a, uint64_t batch,
                                 hipStream_t stream) {
     switch (cols) {
+    case 2: return launch_pass_nc<LOG2NF, 2>(first, swap_in, swap_out, a, batch, stream);
+    case 4: return launch_pass_nc<LOG2NF, 4>(first, swap_in, swap_out, a, batch, stream);
     case 8: return launch_pass_nc<LOG2NF, 8>(first, swap_in, swap_out, a, batch, stream);
     case 16: return launch_pass_nc<LOG2NF, 16>(first, swap_in, swap_out, a, batch, stream);
     case 32: return launch_pass_nc<LOG2NF, 32>(first, swap_in, swap_out, a, batch, stream);
@@ -81,6 +83,8 @@ hipError_t launch_pass(int log2nf, int cols, bool first, bool swap_in, bool swap
     case 8: return launch_pass_n<8>(cols, first, swap_in, swap_out, a, batch, stream);
     case 9: return launch_pass_n<9>(cols, first, swap_in, swap_out, a, batch, stream);
     case 10: return launch_pass_n<10>(cols, first, swap_in, swap_out, a, batch, stream);
+    case 11: return launch_pass_n<11>(cols, first, swap_in, swap_out, a, batch, stream);
+    case 12: return launch_pass_n<12>(cols, first, swap_in, swap_out, a, batch, stream);
     default: return hipErrorInvalidValue;
     }
 }

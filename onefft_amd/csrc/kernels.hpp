@@ -51,19 +51,27 @@ hipError_t launch_batch(int log2n, bool inverse, const BatchArgs &a, BatchVarian
 
 constexpr int kMaxPasses = 4;
 constexpr int kPassMinLog2 = 4;   // smallest sub-transform of a pass
-constexpr int kPassMaxLog2 = 10;  // largest sub-transform of a pass
+constexpr int kPassMaxLog2 = 12;  // largest sub-transform of a pass
 
 // Points per thread of the pass kernel for a 2^log2nf sub-transform, and the columns per
 // workgroup: C adjacent columns make every global access a run of C complex values (C * 8
-// bytes). The default C and the choices a plan may make (LDS of C columns <= 72 KiB).
+// bytes). The default C and the choices a plan may make (LDS of C columns <= 72 KiB). The
+// 2^11- and 2^12-point sub-transforms fit only 4 and 2 columns (32- and 16-byte runs): they exist
+// for two-pass splits of sizes whose data stays in the MALL, where short runs cost less.
 constexpr int pass_log2p(int log2nf) { return log2nf < 4 ? log2nf : 4; }
 constexpr bool pass_cols_ok(int log2nf, int cols) {
-    return (cols == 8 || cols == 16 || cols == 32 || cols == 64) && (cols << log2nf) <= 8192 &&
-           (cols << (log2nf - pass_log2p(log2nf))) <= 1024 && (cols << (log2nf - pass_log2p(log2nf))) >= 64;
+    return (cols == 2 || cols == 4 || cols == 8 || cols == 16 || cols == 32 || cols == 64) &&
+           (cols << log2nf) <= 8192 && (cols << (log2nf - pass_log2p(log2nf))) <= 1024 &&
+           (cols << (log2nf - pass_log2p(log2nf))) >= 64 && (cols >= 8 || log2nf >= 11);
 }
 // default: 16 columns (128-byte runs) where that is allowed, else the nearest allowed count
 constexpr int pass_cols(int log2nf) {
-    return pass_cols_ok(log2nf, 16) ? 16 : pass_cols_ok(log2nf, 32) ? 32 : pass_cols_ok(log2nf, 64) ? 64 : 8;
+    return pass_cols_ok(log2nf, 16)   ? 16
+           : pass_cols_ok(log2nf, 32) ? 32
+           : pass_cols_ok(log2nf, 64) ? 64
+           : pass_cols_ok(log2nf, 8)  ? 8
+           : pass_cols_ok(log2nf, 4)  ? 4
+                                      : 2;
 }
 constexpr int pass_wg(int log2nf, int cols) { return cols << (log2nf - pass_log2p(log2nf)); }
 

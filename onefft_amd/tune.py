@@ -7,7 +7,8 @@ tuning table the library reads at plan creation (tuning.hip).
   padded or swizzled LDS, and, where the size has such stages, FP64-base twiddles at levels 1..3:
   for the stages with Ns >= 64, >= 16, or all);
 * larger sizes: every split n = N_1 ... N_p into 2..4 passes with factors 2^6..2^10 (4 passes only
-  from 2^24), each with the default and the widest allowed columns per workgroup;
+  from 2^24; two-pass splits also with 2^11 and 2^12), each with the default and the widest
+  allowed columns per workgroup;
 each in two working-set regimes, because the best choice differs between them: a batch whose
 input plus output fit the 256 MiB Infinity Cache (16 n batch = 256 MiB up to 2^20 points; one
 transform from 2^21 points, where a single large transform is the common use and has too few tiles
@@ -40,18 +41,19 @@ def pass_cols_ok(log2nf: int, cols: int) -> bool:
     """The kernels' rule (kernels.hpp: pass_cols_ok)."""
     lp = min(log2nf, 4)
     threads = cols << (log2nf - lp)
-    return cols in (8, 16, 32, 64) and (cols << log2nf) <= 8192 and 64 <= threads <= 1024
+    return (cols in (2, 4, 8, 16, 32, 64) and (cols << log2nf) <= 8192 and 64 <= threads <= 1024
+            and (cols >= 8 or log2nf >= 11))
 
 
 def default_cols(log2nf: int) -> int:
-    for c in (16, 32, 64):
+    for c in (16, 32, 64, 8, 4):
         if pass_cols_ok(log2nf, c):
             return c
-    return 8
+    return 2
 
 
 def widest_cols(log2nf: int) -> int:
-    return max(c for c in (8, 16, 32, 64) if pass_cols_ok(log2nf, c))
+    return max(c for c in (2, 4, 8, 16, 32, 64) if pass_cols_ok(log2nf, c))
 
 
 def tw64_levels(log2n: int):
@@ -72,11 +74,14 @@ def tw64_levels(log2n: int):
 
 
 def splits(L: int):
+    """Factors 2^6..2^10 for 2..4 passes (4 only from 2^24); two-pass splits may also use 2^11 and
+    2^12, whose tiles hold only 4 and 2 columns (for sizes whose data stays in the MALL)."""
     out = []
     for p in (2, 3, 4):
         if p == 4 and L < 24:
             continue
-        for combo in itertools.product(range(6, 11), repeat=p):
+        top = 13 if p == 2 else 11
+        for combo in itertools.product(range(6, top), repeat=p):
             if sum(combo) == L:
                 out.append(list(combo))
     return out

@@ -52,7 +52,8 @@ def test_table_choices_are_used_and_correct(tmp_path):
 def test_malformed_table_is_ignored(tmp_path):
     run_with_table(tmp_path, "12 single garbage\n"
                              "16\tmulti\tsplit=6,9;cols=16,16\t1\t1\n"      # factors do not sum to 16
-                             "22\tmulti\tsplit=11,11;cols=8,8\t1\t1\n"      # factor above 2^10
+                             "22\tmulti\tsplit=11,11;cols=8,8\t1\t1\n"      # 8 columns of 2^11 points
+                             "22\tmulti\tsplit=13,9;cols=2,16\t1\t1\n"      # factor above 2^12
                              "99\tsingle\tnt=1,layout=0\t1\t1\n")
 
 
@@ -99,3 +100,42 @@ def test_every_single_pass_variant(monkeypatch):
                     e = O.rel_rms(got[seg], O.fft_f64(x[seg], n, inv))
                     assert e < tol_fp64(L), (L, nt, lay, lv, inv, b, e)
             p.close()
+
+
+@pytest.mark.parametrize("split,cols,batch", [("10,11", "8,4", 3), ("11,11", "4,4", 2), ("11,12", "4,2", 1),
+                                              ("12,11", "2,4", 1), ("12,12", "2,2", 1), ("6,12", "16,2", 2)])
+def test_two_pass_splits_with_wide_factors(monkeypatch, split, cols, batch):
+    """The 2^11- and 2^12-point pass factors (4 and 2 columns per tile) that two-pass splits may
+    use, chosen at plan creation through ONEFFT_PASS_SPLIT / ONEFFT_PASS_COLS: every transform of
+    a small batch, both directions, against the FP64 oracle, and the round trip."""
+    import numpy as np
+    import torch
+
+    import onefft_amd as F
+    from conftest import tol_fp64
+    from oracle import oracle as O
+
+    L = sum(int(v) for v in split.split(","))
+    n = 1 << L
+    print(f"two-pass split {split} cols {cols} n=2^{L} x{batch}", flush=True)
+    monkeypatch.setenv("ONEFFT_PASS_SPLIT", split)
+    monkeypatch.setenv("ONEFFT_PASS_COLS", cols)
+    p = F.Plan(n)
+    monkeypatch.delenv("ONEFFT_PASS_SPLIT")
+    monkeypatch.delenv("ONEFFT_PASS_COLS")
+    s = torch.cuda.current_stream().cuda_stream
+    x = O.normal(70 + L, n * batch)
+    d = torch.from_numpy(x).cuda()
+    y = torch.empty_like(d)
+    z = torch.empty_like(d)
+    p.batch_ptr(d.data_ptr(), y.data_ptr(), batch, n, n, False, s)
+    p.batch_ptr(y.data_ptr(), z.data_ptr(), batch, n, n, True, s)
+    torch.cuda.synchronize()
+    yh = y.cpu().numpy()
+    want = O.fft_f64(x, n, False, batch=batch)
+    for b in range(batch):
+        seg = slice(2 * n * b, 2 * n * (b + 1))
+        assert O.rel_rms(yh[seg], want[seg]) < tol_fp64(L), (split, b)
+    rt = float(np.linalg.norm(z.cpu().numpy() / n - x) / np.linalg.norm(x))
+    assert rt < 2 * tol_fp64(L), (split, rt)
+    p.close()

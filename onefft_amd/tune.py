@@ -7,8 +7,8 @@ tuning table the library reads at plan creation (tuning.hip).
   padded or swizzled LDS, and, where the size has such stages, FP64-base twiddles at levels 1..3:
   for the stages with Ns >= 64, >= 16, or all);
 * larger sizes: every split n = N_1 ... N_p into 2..4 passes with factors 2^6..2^10 (4 passes only
-  from 2^24; two-pass splits also with 2^11 and 2^12), each with the default and the widest
-  allowed columns per workgroup;
+  from 2^24; two- and three-pass splits also with 2^11 and 2^12), each with the default and the
+  widest allowed columns per workgroup;
 each in two working-set regimes, because the best choice differs between them: a batch whose
 input plus output fit the 256 MiB Infinity Cache (16 n batch = 256 MiB up to 2^20 points; one
 transform from 2^21 points, where a single large transform is the common use and has too few tiles
@@ -74,13 +74,13 @@ def tw64_levels(log2n: int):
 
 
 def splits(L: int):
-    """Factors 2^6..2^10 for 2..4 passes (4 only from 2^24); two-pass splits may also use 2^11 and
-    2^12, whose tiles hold only 4 and 2 columns (for sizes whose data stays in the MALL)."""
+    """Factors 2^6..2^10 for 2..4 passes (4 only from 2^24); two- and three-pass splits may also
+    use 2^11 and 2^12, whose tiles hold only 4 and 2 columns."""
     out = []
     for p in (2, 3, 4):
         if p == 4 and L < 24:
             continue
-        top = 13 if p == 2 else 11
+        top = 13 if p <= 3 else 11
         for combo in itertools.product(range(6, top), repeat=p):
             if sum(combo) == L:
                 out.append(list(combo))

@@ -103,11 +103,13 @@ def test_every_single_pass_variant(monkeypatch):
 
 
 @pytest.mark.parametrize("split,cols,batch", [("10,11", "8,4", 3), ("11,11", "4,4", 2), ("11,12", "4,2", 1),
-                                              ("12,11", "2,4", 1), ("12,12", "2,2", 1), ("6,12", "16,2", 2)])
-def test_two_pass_splits_with_wide_factors(monkeypatch, split, cols, batch):
-    """The 2^11- and 2^12-point pass factors (4 and 2 columns per tile) that two-pass splits may
-    use, chosen at plan creation through ONEFFT_PASS_SPLIT / ONEFFT_PASS_COLS: every transform of
-    a small batch, both directions, against the FP64 oracle, and the round trip."""
+                                              ("12,11", "2,4", 1), ("12,12", "2,2", 1), ("6,12", "16,2", 2),
+                                              ("6,6,11", "16,16,4", 1), ("6,12,6", "16,2,16", 2),
+                                              ("11,6,6", "4,16,16", 1)])
+def test_splits_with_wide_factors(monkeypatch, split, cols, batch):
+    """The 2^11- and 2^12-point pass factors (4 and 2 columns per tile) that two- and three-pass
+    splits may use, chosen at plan creation through ONEFFT_PASS_SPLIT / ONEFFT_PASS_COLS: every
+    transform of a small batch, forward, against the FP64 oracle, and the round trip."""
     import numpy as np
     import torch
 

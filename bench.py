@@ -42,8 +42,8 @@ def parse():
                     help="only the timed transform launches plus one 1 GiB device copy (for counter runs)")
     ap.add_argument("--no-extra", action="store_true", help="skip the other workloads")
     ap.add_argument("--single-transform", type=int, default=None, metavar="LOG2N",
-                    help="instead of the benchmark: 3 forward transforms of one 2^LOG2N-point input (random, "
-                         "resident in HBM) and one device copy of the same bytes, for counter runs")
+                    help="instead of the benchmark: 20 forward transforms of one 2^LOG2N-point input (random, "
+                         "resident in HBM) and one device copy of the same bytes, for traces and counter runs")
     ap.add_argument("--ragged-only", action="store_true",
                     help="instead of the benchmark: 3 one-launch executions of the 4 GiB ragged prefix (seed-3 items, "
                          "random input resident in HBM), for counter runs")
@@ -524,12 +524,12 @@ def main():
         y = torch.empty_like(x)
         p = F.Plan(n)
         sp = torch.cuda.current_stream().cuda_stream
-        for _ in range(3):
+        for _ in range(20):  # the first few run while the clocks leave idle
             p.batch_ptr(x.data_ptr(), y.data_ptr(), 1, n, n, False, sp)
         z = torch.empty_like(x)
         z.copy_(x)  # a plain copy of known size: 8 n bytes read, 8 n bytes written
         torch.cuda.synchronize()
-        print(json.dumps({"single_transform": args.single_transform, "launches": 3}), flush=True)
+        print(json.dumps({"single_transform": args.single_transform, "launches": 20}), flush=True)
         return
     if args.ragged_only:
         items = F.ragged_items(3, 200000, max_bytes=4 << 30)

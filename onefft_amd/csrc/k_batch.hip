@@ -14,7 +14,28 @@
 namespace onefft {
 
 template <int LOG2N>
+static hipError_t launch_small(bool inverse, bool nt, const BatchArgs &a, hipStream_t stream) {
+    const uint64_t total = a.batch << LOG2N;
+    const uint64_t blocks = (total + kSmallBlock - 1) / kSmallBlock;
+    if (blocks > 0x7fffffffu) return hipErrorInvalidValue;
+    const dim3 grid((uint32_t)blocks);
+    if (inverse) {
+        if (nt) hipLaunchKernelGGL((fft_small_kernel<LOG2N, true, true>), grid, dim3(256), 0, stream, a.in, a.out, total);
+        else hipLaunchKernelGGL((fft_small_kernel<LOG2N, true, false>), grid, dim3(256), 0, stream, a.in, a.out, total);
+    } else {
+        if (nt) hipLaunchKernelGGL((fft_small_kernel<LOG2N, false, true>), grid, dim3(256), 0, stream, a.in, a.out, total);
+        else hipLaunchKernelGGL((fft_small_kernel<LOG2N, false, false>), grid, dim3(256), 0, stream, a.in, a.out, total);
+    }
+    return hipGetLastError();
+}
+
+template <int LOG2N>
 static hipError_t launch_batch_n(bool inverse, const BatchArgs &a, BatchVariant v, hipStream_t stream) {
+    if constexpr (LOG2N <= 4) {
+        // contiguous batches of up to 16 points: coalesced blocks (k_batch.hpp: fft_small_kernel)
+        if (a.batch > 0 && a.in_dist == (1u << LOG2N) && a.out_dist == (1u << LOG2N) && a.batch <= (~0ull >> 5))
+            return launch_small<LOG2N>(inverse, v.nt, a, stream);
+    }
     constexpr int PER = batch_per_wg(LOG2N);
     const uint64_t blocks = (a.batch + PER - 1) / PER;
     if (blocks == 0) return hipSuccess;

@@ -48,6 +48,56 @@ inline void launch_variant(bool inverse, const BatchArgs &a, dim3 grid, hipStrea
                            a.in, a.out, a.tw, a.tw64, a.in_dist, a.out_dist, a.batch);
 }
 
+// Contiguous batches of transforms of n <= 16 points (in_dist = out_dist = n; the reference's small
+// kernels, src/onefft_avx512.inc n32/n64/n128 and its 1, 2 and 4-point closed forms,
+// src/onefft.c:158-181). Each 256-thread workgroup moves one block of 4096 values with coalesced
+// 8-byte accesses (thread t: values t + 256 r), exchanges them through LDS so that each thread
+// holds 16 / n whole transforms, runs the register DFTs, and exchanges back before the coalesced
+// stores. (The general kernel gives each thread one transform of n points: a wavefront's accesses
+// then touch 64 lines 8 n bytes apart.) Only the last workgroup's block can be partial; its lanes
+// skip the values past the end.
+constexpr uint32_t kSmallBlock = 4096;
+
+template <int LOG2N, bool INV, bool NT>
+__global__ __launch_bounds__(256) void fft_small_kernel(const cf *__restrict__ in, cf *__restrict__ out,
+                                                         uint64_t total) {
+    constexpr int N = 1 << LOG2N;
+    static_assert(LOG2N <= 4, "up to 16 points: one register DFT per 16 / n values");
+    __shared__ cf lds[LOG2N > 0 ? lds_span(kSmallBlock) : 1];
+    const uint32_t t = threadIdx.x;
+    const uint64_t base = (uint64_t)blockIdx.x * kSmallBlock;
+    const uint64_t left = total - base;
+    const uint32_t count = left < kSmallBlock ? (uint32_t)left : kSmallBlock;
+    const BufIO<NT> io(in + base, out + base);
+    cf v[16];
+    if (count == kSmallBlock) {
+        static_for<0, 16>([&](auto r) { v[r] = io.load(t, decltype(r)::value * 256u); });
+    } else {
+        static_for<0, 16>([&](auto r) {
+            constexpr uint32_t c = decltype(r)::value * 256u;
+            v[r] = t + c < count ? io.load(t, c) : mk(0.f, 0.f);
+        });
+    }
+    if constexpr (LOG2N > 0) {
+        static_for<0, 16>([&](auto r) { lds[lds_pad(t + decltype(r)::value * 256u)] = v[r]; });
+        __syncthreads();
+        // thread t now works on values 16 t .. 16 t + 15 alone: 16 / n whole transforms
+        static_for<0, 16>([&](auto k) { v[k] = maybe_conj<INV>(lds[lds_pad(16 * t + decltype(k)::value)]); });
+        static_for<0, 16 / N>([&](auto q) { dft<N>(&v[decltype(q)::value * N]); });
+        static_for<0, 16>([&](auto k) { lds[lds_pad(16 * t + decltype(k)::value)] = maybe_conj<INV>(v[k]); });
+        __syncthreads();
+        static_for<0, 16>([&](auto r) { v[r] = lds[lds_pad(t + decltype(r)::value * 256u)]; });
+    }
+    if (count == kSmallBlock) {
+        static_for<0, 16>([&](auto r) { io.store(t, decltype(r)::value * 256u, v[r]); });
+    } else {
+        static_for<0, 16>([&](auto r) {
+            constexpr uint32_t c = decltype(r)::value * 256u;
+            if (t + c < count) io.store(t, c, v[r]);
+        });
+    }
+}
+
 // Launches the FP64-base variant `level` (1..3, at most the size's tw64_levels) of the padded
 // kernel for 2^log2n points, log2n <= kMaxSinglePassLog2 (k_batch_tw.hip).
 hipError_t launch_batch_tw64(int log2n, int level, bool nt, bool inverse, const BatchArgs &a, dim3 grid,

@@ -67,7 +67,7 @@ __device__ __forceinline__ void single_finish(cf (&v)[Single<LOG2N>::P], const I
         constexpr bool BASE64 = stage_uses_tw64(LOG2N, LOG2P, s, TW64);
         constexpr uint32_t TWO64 = stage_tw64_offset(LOG2N, LOG2P, s);
 
-        if constexpr (s > 0) {
+        if constexpr (s > 0 && LAYOUT != kHalves) {  // (kHalves: the previous exchange loaded v)
             static_for<0, BF>([&](auto q) {
                 static_for<0, R>([&](auto r) {
                     constexpr int qq = decltype(q)::value, rr = decltype(r)::value;
@@ -109,6 +109,43 @@ __device__ __forceinline__ void single_finish(cf (&v)[Single<LOG2N>::P], const I
                     });
                 });
             }
+        } else if constexpr (LAYOUT == kHalves) {
+            // the exchange to stage s + 1, real parts first, then imaginary parts, through one
+            // float buffer: write (this stage's pattern), barrier, read (the next stage's), barrier
+            float *Lf = reinterpret_cast<float *>(L);
+            constexpr int R1 = 1 << stage_log2r(LOG2N, LOG2P, s + 1);
+            constexpr int BF1 = P / R1;
+            constexpr uint32_t STRIDE1 = N / R1;
+            auto put = [&](auto part) {
+                static_for<0, BF>([&](auto q) {
+                    constexpr int qq = decltype(q)::value;
+                    const uint32_t b = j + T * qq;
+                    const uint32_t base = ((b >> LNS) << (LNS + LR)) + (b & (NS - 1));
+                    static_for<0, R>([&](auto m) {
+                        constexpr int mm = decltype(m)::value;
+                        Lf[lds_at<kPadded>(base, mm * NS)] = decltype(part)::value ? v[qq * R + mm].i : v[qq * R + mm].r;
+                    });
+                });
+            };
+            float re[P];
+            if constexpr (s > 0) __syncthreads();  // every read of the previous exchange is done
+            put(std::false_type{});
+            __syncthreads();
+            static_for<0, BF1>([&](auto q) {
+                static_for<0, R1>([&](auto r) {
+                    constexpr int qq = decltype(q)::value, rr = decltype(r)::value;
+                    re[qq * R1 + rr] = Lf[lds_at<kPadded>(j + T * qq, rr * STRIDE1)];
+                });
+            });
+            __syncthreads();
+            put(std::true_type{});
+            __syncthreads();
+            static_for<0, BF1>([&](auto q) {
+                static_for<0, R1>([&](auto r) {
+                    constexpr int qq = decltype(q)::value, rr = decltype(r)::value;
+                    v[qq * R1 + rr] = mk(re[qq * R1 + rr], Lf[lds_at<kPadded>(j + T * qq, rr * STRIDE1)]);
+                });
+            });
         } else {
             if constexpr (s > 0) __syncthreads();  // every read of this stage is done
             static_for<0, BF>([&](auto q) {

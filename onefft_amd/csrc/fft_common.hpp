@@ -239,13 +239,17 @@ __host__ __device__ constexpr int tw64_levels(int log2n, int log2p) {
 
 // ---- LDS layout ---------------------------------------------------------------------------
 //
-// Two layouts of a transform's values in LDS, chosen per kernel variant:
+// Three layouts of a transform's values in LDS, chosen per kernel variant:
 //   kPadded   one complex of padding after every 16: stage accesses that stride by powers of two
 //             spread over the banks, at 1/16 extra LDS;
 //   kSwizzled no padding; the low 4 index bits are XORed with bits 4..7 and 8..11, a bijection
 //             that spreads the same accesses over the banks and keeps a 4096-point transform at
-//             exactly 32 KiB (one more workgroup per CU).
-enum LdsLayout : int { kPadded = 0, kSwizzled = 1 };
+//             exactly 32 KiB (one more workgroup per CU);
+//   kHalves   the padded index over floats: a stage exchange moves the real parts, then the
+//             imaginary parts, through one buffer of n + n/16 floats, half the padded layout's
+//             LDS (twice the workgroups per CU) for twice the barriers.
+// lds_extent is in complex values (kHalves: enough of them to hold the floats).
+enum LdsLayout : int { kPadded = 0, kSwizzled = 1, kHalves = 2 };
 
 template <int LAYOUT = kPadded>
 __host__ __device__ constexpr uint32_t lds_index(uint32_t i) {
@@ -254,7 +258,7 @@ __host__ __device__ constexpr uint32_t lds_index(uint32_t i) {
 }
 template <int LAYOUT = kPadded>
 __host__ __device__ constexpr uint32_t lds_extent(uint32_t n) {
-    return LAYOUT == kSwizzled ? n : n + (n >> 4);
+    return LAYOUT == kSwizzled ? n : LAYOUT == kHalves ? (n + (n >> 4) + 1) / 2 : n + (n >> 4);
 }
 // lds_index(b + c) with the part that depends on b computed once: for a base b computed at run
 // time and a compile-time step c, the padded index is lds_index(b) + c + (c >> 4) whenever the low
@@ -265,7 +269,7 @@ __host__ __device__ constexpr uint32_t lds_extent(uint32_t n) {
 template <int LAYOUT = kPadded>
 __host__ __device__ constexpr uint32_t lds_at(uint32_t b, uint32_t c) {
     if constexpr (LAYOUT == kSwizzled) return lds_index<kSwizzled>(b + c);
-    else return lds_index<kPadded>(b) + c + (c >> 4);
+    else return lds_index<kPadded>(b) + c + (c >> 4);  // kPadded, and kHalves in float units
 }
 // the padded layout, used by the kernels without a layout choice
 __host__ __device__ constexpr uint32_t lds_pad(uint32_t i) { return lds_index<kPadded>(i); }

@@ -46,8 +46,17 @@ static hipError_t launch_batch_n(bool inverse, const BatchArgs &a, BatchVariant 
     // last one that changes this size's kernel runs that one (k_batch_tw.hip)
     constexpr int TOP = tw64_levels(LOG2N, batch_log2p(LOG2N));
     const int level = v.tw64 < TOP ? v.tw64 : TOP;
-    if (level > 0) return launch_batch_tw64(LOG2N, level, v.nt, inverse, a, grid, stream);
-    const bool swz = USES_LDS && v.layout == kSwizzled;
+    // the halves layout exists for the sizes whose LDS limits the workgroups per CU (kernels.hpp)
+    const int layout = !USES_LDS ? kPadded : (v.layout == kHalves && LOG2N < kHalvesMinLog2) ? kPadded : v.layout;
+    if (level > 0) return launch_batch_tw64(LOG2N, level, layout == kHalves, v.nt, inverse, a, grid, stream);
+    if constexpr (LOG2N >= kHalvesMinLog2) {
+        if (layout == kHalves) {
+            if (v.nt) launch_variant<LOG2N, true, kHalves>(inverse, a, grid, stream);
+            else launch_variant<LOG2N, false, kHalves>(inverse, a, grid, stream);
+            return hipGetLastError();
+        }
+    }
+    const bool swz = layout == kSwizzled;
     if (v.nt) {
         if (swz) launch_variant<LOG2N, true, kSwizzled>(inverse, a, grid, stream);
         else launch_variant<LOG2N, true, kPadded>(inverse, a, grid, stream);

@@ -98,9 +98,10 @@ __global__ __launch_bounds__(256) void fft_small_kernel(const cf *__restrict__ i
     }
 }
 
-// Launches the FP64-base variant `level` (1..3, at most the size's tw64_levels) of the padded
-// kernel for 2^log2n points, log2n <= kMaxSinglePassLog2 (k_batch_tw.hip).
-hipError_t launch_batch_tw64(int log2n, int level, bool nt, bool inverse, const BatchArgs &a, dim3 grid,
+// Launches the FP64-base variant `level` (1..3, at most the size's tw64_levels) of the padded (or,
+// with `halves` and log2n >= kHalvesMinLog2, the halves-layout) kernel for 2^log2n points,
+// log2n <= kMaxSinglePassLog2 (k_batch_tw.hip).
+hipError_t launch_batch_tw64(int log2n, int level, bool halves, bool nt, bool inverse, const BatchArgs &a, dim3 grid,
                              hipStream_t stream);
 
 }  // namespace onefft

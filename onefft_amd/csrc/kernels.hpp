@@ -35,9 +35,10 @@ struct BatchArgs {
 };
 
 // Kernel variant of the single-pass transform: nontemporal global access, the LDS layout
-// (fft_common.hpp: kPadded / kSwizzled), and the level of FP64-base twiddles (0..3,
-// fft_common.hpp: stage_uses_tw64; with the padded layout only). The plan's tables serve every
-// variant.
+// (fft_common.hpp: kPadded / kSwizzled / kHalves, the last from 2^kHalvesMinLog2 points), and the
+// level of FP64-base twiddles (0..3, fft_common.hpp: stage_uses_tw64; with the padded and halves
+// layouts). The plan's tables serve every variant.
+constexpr int kHalvesMinLog2 = 12;
 struct BatchVariant {
     bool nt = true;
     int layout = 0;

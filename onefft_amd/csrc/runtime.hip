@@ -175,7 +175,7 @@ int choose_split_factors(unsigned log2n, int regime, int *factors) {
 }
 
 // Single-pass kernel variant: the tuning table's choice, else the BatchVariant defaults
-// (kernels.hpp). ONEFFT_BATCH_VARIANT="nt,layout[,tw64]" (nt and layout 0/1, tw64 the FP64-base
+// (kernels.hpp). ONEFFT_BATCH_VARIANT="nt,layout[,tw64]" (nt 0/1, layout 0..2, tw64 the FP64-base
 // level 0..3, 1 when omitted) overrides the table, for measurement.
 onefft::BatchVariant choose_variant(unsigned log2n, int regime) {
     onefft::BatchVariant v;
@@ -185,7 +185,7 @@ onefft::BatchVariant choose_variant(unsigned log2n, int regime) {
         int nt = 1, layout = 0, tw64 = 1;
         if (std::sscanf(env, "%d,%d,%d", &nt, &layout, &tw64) >= 2) {
             v.nt = nt != 0;
-            v.layout = layout ? 1 : 0;
+            v.layout = layout < 0 ? 0 : layout > 2 ? 2 : layout;
             v.tw64 = tw64 < 0 ? 0 : tw64 > 3 ? 3 : tw64;
         }
     }

@@ -2,7 +2,7 @@
 // multi-pass split and columns per workgroup that measured fastest on this GPU.
 //
 // File: tab-separated lines "log2n  kind  params  gb_s  batch", '#' comments, where kind is
-// "single" (params "nt=0|1,layout=0|1[,tw64=0..3]") or "multi" (params "split=a,b[,c[,d]];cols=x,y[,..]",
+// "single" (params "nt=0|1,layout=0..2[,tw64=0..3]") or "multi" (params "split=a,b[,c[,d]];cols=x,y[,..]",
 // log2 N_1 first). The batch the line was measured at sets its working-set regime
 // (runtime.hpp: tune_regime); a size may have a line per regime. Written by `python -m onefft_amd.tune`; read once, at the first plan creation,
 // from $ONEFFT_TUNE_FILE or tune_gfx950.tsv next to the library. Plan creation stays O(n) with no
@@ -65,7 +65,7 @@ void parse_line(const char *line) {
         if (std::sscanf(params, "nt=%d,layout=%d,tw64=%d", &nt, &layout, &tw64) < 2) return;
         e.kind = TuneEntry::kSingle;
         e.variant.nt = nt != 0;
-        e.variant.layout = layout ? 1 : 0;
+        e.variant.layout = layout < 0 ? 0 : layout > 2 ? 2 : layout;
         e.variant.tw64 = tw64 < 0 ? 0 : tw64 > 3 ? 3 : tw64;
     } else if (!std::strcmp(kind, "multi")) {
         const char *sp = std::strstr(params, "split="), *cp = std::strstr(params, "cols=");

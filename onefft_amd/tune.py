@@ -5,7 +5,8 @@ tuning table the library reads at plan creation (tuning.hip).
 
 * sizes up to 2^14 (one LDS pass): the single-pass variants (nontemporal global access on or off,
   padded or swizzled LDS, and, where the size has such stages, FP64-base twiddles at levels 1..3:
-  for the stages with Ns >= 64, >= 16, or all);
+  for the stages with Ns >= 64, >= 16, or all; from 2^12 also the halves LDS layout at every
+  level);
 * larger sizes: every split n = N_1 ... N_p into 2..4 passes with factors 2^6..2^10 (4 passes only
   from 2^24; two- and three-pass splits also with 2^11 and 2^12), each with the default and the
   widest allowed columns per workgroup;
@@ -17,8 +18,9 @@ per pass for the batch's best split), and a batch of 1 GiB in and 1 GiB out (one
 tune_regime).
 
 Each candidate is a plan created under the environment overrides (ONEFFT_BATCH_VARIANT,
-ONEFFT_PASS_SPLIT, ONEFFT_PASS_COLS) and timed with HIP events: the median of 5 launches after 2
-warm-up launches. All candidates of a size give results within FP32 rounding of each other; the
+ONEFFT_PASS_SPLIT, ONEFFT_PASS_COLS) and timed with HIP events: after 3 warm-up launches, the
+fastest of 5 samples of 10 back-to-back launches (single launches timed alone varied by more than
+the 2-5 % that separates the best variants). All candidates of a size give results within FP32 rounding of each other; the
 tuner checks each against the first candidate (rel-L2 <= 3e-7 log2 n). The fastest is written.
 The table is read once per process, so this module sets ONEFFT_TUNE_FILE to a missing file before
 the first plan: the candidates are measured without any table.
@@ -29,12 +31,15 @@ import argparse
 import itertools
 import math
 import os
-import statistics
 import sys
 import time
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DEFAULT_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_gfx950.tsv")
+
+
+HALVES_MIN_LOG2 = 12  # kernels.hpp: kHalvesMinLog2
+GROUP = 10  # launches per timed sample: back to back, as batches run, so a 2 % difference shows
 
 
 def pass_cols_ok(log2nf: int, cols: int) -> bool:
@@ -90,8 +95,10 @@ def splits(L: int):
 def candidates(L: int, single: bool):
     if single:
         combos = [(nt, lay, 0) for nt in (1, 0) for lay in (0, 1)]
-        # FP64-base levels that change this size's kernel (padded layout only)
+        # FP64-base levels that change this size's kernel (padded layout)
         combos += [(nt, 0, level) for level in tw64_levels(L) for nt in (1, 0)]
+        if L >= HALVES_MIN_LOG2:  # the halves layout, with every level
+            combos += [(nt, 2, level) for level in [0] + tw64_levels(L) for nt in (1, 0)]
         return [dict(kind="single", params=f"nt={nt},layout={lay},tw64={t64}",
                      env={"ONEFFT_BATCH_VARIANT": f"{nt},{lay},{t64}"}) for nt, lay, t64 in combos]
     out = []
@@ -125,6 +132,7 @@ def tune_size(F, torch, timed, L: int, batch: int, single: bool) -> str:
             if err > 3e-7 * max(L, 1):
                 raise SystemExit(f"log2n={L} {cand['params']}: result differs from the first candidate ({err:.3g})")
         results.append((16.0 * n * batch / (ms * 1e-3) / 1e9, cand["kind"], cand["params"]))
+        print(f"  log2n={L} batch={batch} {cand['params']}: {results[-1][0]:.0f} GB/s", flush=True)
         plan.close()
         for k in cand["env"]:
             os.environ.pop(k, None)
@@ -154,21 +162,31 @@ def main(argv=None):
 
     def timed(plan, x, y, batch, n):
         run = lambda: plan.batch_ptr(x.data_ptr(), y.data_ptr(), batch, n, n, False, sp)
-        for _ in range(2):
+        for _ in range(3):
             run()
         torch.cuda.synchronize()
         ts = []
         for _ in range(5):
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record(stream)
-            run()
+            for _ in range(GROUP):
+                run()
             e1.record(stream)
             torch.cuda.synchronize()
-            ts.append(e0.elapsed_time(e1))
-        return statistics.median(ts)
+            ts.append(e0.elapsed_time(e1) / GROUP)
+        return min(ts)
 
     lines = []
     props = torch.cuda.get_device_properties(0)
+    header = [
+        "# OneFFT gfx950 tuning table: log2n, kind, params, measured GB/s (16 n batch / time), batch",
+        f"# written by python -m onefft_amd.tune on {props.name} ({time.strftime('%Y-%m-%d')})",
+    ]
+
+    def write(lines):
+        with open(args.out, "w") as f:
+            f.write("\n".join(header + lines) + "\n")
+
     for L in range(args.min, args.max + 1):
         n = 1 << L
         single = L <= 14
@@ -183,12 +201,7 @@ def main(argv=None):
                     continue
             lines.append(tune_size(F, torch, timed, L, batch, single))
             torch.cuda.empty_cache()
-    header = [
-        "# OneFFT gfx950 tuning table: log2n, kind, params, measured GB/s (16 n batch / time), batch",
-        f"# written by python -m onefft_amd.tune on {props.name} ({time.strftime('%Y-%m-%d')})",
-    ]
-    with open(args.out, "w") as f:
-        f.write("\n".join(header + lines) + "\n")
+            write(lines)  # after every size, so an interrupted run keeps what it measured
     print(f"wrote {args.out}", flush=True)
 
 

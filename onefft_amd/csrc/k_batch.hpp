@@ -98,6 +98,43 @@ __global__ __launch_bounds__(256) void fft_small_kernel(const cf *__restrict__ i
     }
 }
 
+// Launches the FP64-base variant `level` (1..3, at most the size's tw64_levels) of the kernel for
+// 2^LOG2N points, padded LDS or, with `halves` from 2^kHalvesMinLog2 points, the halves layout.
+template <int LOG2N>
+inline hipError_t launch_tw64_n(int level, bool halves, bool nt, bool inverse, const BatchArgs &a, dim3 grid,
+                         hipStream_t stream) {
+    constexpr int TOP = tw64_levels(LOG2N, batch_log2p(LOG2N));
+    if constexpr (TOP == 0) {
+        return hipErrorInvalidValue;
+    } else {
+        auto run = [&](auto lv) {
+            constexpr int L = decltype(lv)::value;
+            if constexpr (LOG2N >= kHalvesMinLog2) {
+                if (halves) {
+                    if (nt) launch_variant<LOG2N, true, kHalves, L>(inverse, a, grid, stream);
+                    else launch_variant<LOG2N, false, kHalves, L>(inverse, a, grid, stream);
+                    return;
+                }
+            }
+            if (nt) launch_variant<LOG2N, true, kPadded, L>(inverse, a, grid, stream);
+            else launch_variant<LOG2N, false, kPadded, L>(inverse, a, grid, stream);
+        };
+        if (level < 1 || level > TOP) return hipErrorInvalidValue;
+        if (level == 1) run(std::integral_constant<int, 1>{});
+        if constexpr (TOP >= 2) {
+            if (level == 2) run(std::integral_constant<int, 2>{});
+        }
+        if constexpr (TOP >= 3) {
+            if (level == 3) run(std::integral_constant<int, 3>{});
+        }
+        return hipGetLastError();
+    }
+}
+
+// 2^11..2^14 points (k_batch_tw_large.hip), called by launch_batch_tw64.
+hipError_t launch_batch_tw64_large(int log2n, int level, bool halves, bool nt, bool inverse, const BatchArgs &a,
+                                   dim3 grid, hipStream_t stream);
+
 // Launches the FP64-base variant `level` (1..3, at most the size's tw64_levels) of the padded (or,
 // with `halves` and log2n >= kHalvesMinLog2, the halves-layout) kernel for 2^log2n points,
 // log2n <= kMaxSinglePassLog2 (k_batch_tw.hip).

@@ -38,7 +38,7 @@ struct BatchArgs {
 // (fft_common.hpp: kPadded / kSwizzled / kHalves, the last from 2^kHalvesMinLog2 points), and the
 // level of FP64-base twiddles (0..3, fft_common.hpp: stage_uses_tw64; with the padded and halves
 // layouts). The plan's tables serve every variant.
-constexpr int kHalvesMinLog2 = 12;
+constexpr int kHalvesMinLog2 = 8;
 struct BatchVariant {
     bool nt = true;
     int layout = 0;

@@ -5,7 +5,7 @@ tuning table the library reads at plan creation (tuning.hip).
 
 * sizes up to 2^14 (one LDS pass): the single-pass variants (nontemporal global access on or off,
   padded or swizzled LDS, and, where the size has such stages, FP64-base twiddles at levels 1..3:
-  for the stages with Ns >= 64, >= 16, or all; from 2^12 also the halves LDS layout at every
+  for the stages with Ns >= 64, >= 16, or all; from 2^8 also the halves LDS layout at every
   level);
 * larger sizes: every split n = N_1 ... N_p into 2..4 passes with factors 2^6..2^10 (4 passes only
   from 2^24; two- and three-pass splits also with 2^11 and 2^12), each with the default and the
@@ -38,7 +38,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DEFAULT_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_gfx950.tsv")
 
 
-HALVES_MIN_LOG2 = 12  # kernels.hpp: kHalvesMinLog2
+HALVES_MIN_LOG2 = 8  # kernels.hpp: kHalvesMinLog2
 GROUP = 10  # launches per timed sample: back to back, as batches run, so a 2 % difference shows
 
 

@@ -87,7 +87,7 @@ def test_every_single_pass_variant(monkeypatch):
         d = torch.from_numpy(x).cuda()
         variants = [(nt, lay, 0) for nt in (0, 1) for lay in (0, 1)]
         variants += [(nt, 0, lv) for lv in tw64_levels(L) for nt in (0, 1)]
-        if L >= 12:  # the halves layout (kernels.hpp: kHalvesMinLog2)
+        if L >= 8:  # the halves layout (kernels.hpp: kHalvesMinLog2)
             variants += [(nt, 2, lv) for lv in [0] + tw64_levels(L) for nt in (0, 1)]
         for nt, lay, lv in variants:
             monkeypatch.setenv("ONEFFT_BATCH_VARIANT", f"{nt},{lay},{lv}")

@@ -175,12 +175,14 @@ __device__ __forceinline__ void single_body(const IO &io, bool live, const cf *_
 //
 // Columns tile*C .. tile*C + C-1 of the pass described by `a` (see k_pass.hip for the index
 // scheme), on the transform at src/dst. Thread t of the C * 2^LOG2NF / P threads handles column
-// t % C. lds holds C * pass_span(LOG2NF) values.
+// t % C. lds holds C * pass_span(LOG2NF) complex values (kPadded) or floats (kHalves: the stage
+// exchanges move real and imaginary parts in turn, as in single_finish).
 __host__ __device__ constexpr uint32_t pass_span(int log2nf) { return lds_span(1u << log2nf) | 1; }
 
-template <int LOG2NF, int C, bool FIRST, bool SWAP_IN, bool SWAP_OUT>
+template <int LOG2NF, int C, bool FIRST, bool SWAP_IN, bool SWAP_OUT, int LAYOUT = kPadded>
 __device__ __forceinline__ void pass_body(const PassArgs &a, const cf *__restrict__ src, cf *__restrict__ dst,
                                           uint32_t tile, cf *lds, uint32_t t) {
+    static_assert(LAYOUT == kPadded || LAYOUT == kHalves, "pass kernels use the padded or the halves layout");
     constexpr int LOG2P = pass_log2p(LOG2NF);
     constexpr int NF = 1 << LOG2NF;
     constexpr int P = 1 << LOG2P;
@@ -191,7 +193,9 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, const cf *__restric
     const uint32_t c = t % C;
     const uint32_t j = t / C;
     const uint32_t col = tile * C + c;
+    // a column's region: SPAN complex values, or with the halves layout SPAN floats
     cf *L = lds + c * SPAN;
+    float *Lf = reinterpret_cast<float *>(lds) + c * SPAN;
 
     uint32_t in_base, in_step, out_base, out_step, low = 0;
     if constexpr (FIRST) {
@@ -252,7 +256,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, const cf *__restric
                     if constexpr (qq + 1 < BF) wq = mul64(wq, step_q);
                 });
             }
-        } else {
+        } else if constexpr (LAYOUT != kHalves) {  // (kHalves: the previous exchange loaded v)
             static_for<0, BF>([&](auto q) {
                 static_for<0, R>([&](auto r) {
                     constexpr int qq = decltype(q)::value, rr = decltype(r)::value;
@@ -284,6 +288,42 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, const cf *__restric
                 static_for<0, R>([&](auto m) {
                     constexpr int qq = decltype(q)::value, mm = decltype(m)::value;
                     dst[out_base + (j + T * qq + mm * STRIDE) * out_step] = maybe_swap<SWAP_OUT>(v[qq * R + mm]);
+                });
+            });
+        } else if constexpr (LAYOUT == kHalves) {
+            // the exchange to stage s + 1 through one float buffer, real parts, then imaginary
+            // parts (as in single_finish)
+            constexpr int R1 = 1 << stage_log2r(LOG2NF, LOG2P, s + 1);
+            constexpr int BF1 = P / R1;
+            constexpr uint32_t STRIDE1 = NF / R1;
+            auto put = [&](auto part) {
+                static_for<0, BF>([&](auto q) {
+                    constexpr int qq = decltype(q)::value;
+                    const uint32_t b = j + T * qq;
+                    const uint32_t base = ((b >> LNS) << (LNS + LR)) + (b & (NS - 1));
+                    static_for<0, R>([&](auto m) {
+                        constexpr int mm = decltype(m)::value;
+                        Lf[lds_at(base, mm * NS)] = decltype(part)::value ? v[qq * R + mm].i : v[qq * R + mm].r;
+                    });
+                });
+            };
+            float re[P];
+            if constexpr (s > 0) __syncthreads();  // every read of the previous exchange is done
+            put(std::false_type{});
+            __syncthreads();
+            static_for<0, BF1>([&](auto q) {
+                static_for<0, R1>([&](auto r) {
+                    constexpr int qq = decltype(q)::value, rr = decltype(r)::value;
+                    re[qq * R1 + rr] = Lf[lds_at(j + T * qq, rr * STRIDE1)];
+                });
+            });
+            __syncthreads();
+            put(std::true_type{});
+            __syncthreads();
+            static_for<0, BF1>([&](auto q) {
+                static_for<0, R1>([&](auto r) {
+                    constexpr int qq = decltype(q)::value, rr = decltype(r)::value;
+                    v[qq * R1 + rr] = mk(re[qq * R1 + rr], Lf[lds_at(j + T * qq, rr * STRIDE1)]);
                 });
             });
         } else {

@@ -97,8 +97,8 @@ struct PassArgs {
 
 // One pass of a multi-pass transform: first = reads the input (out of place), else in place on
 // `out`. swap_in / swap_out exchange re and im on load / store (the inverse). cols: columns per
-// workgroup (pass_cols_ok).
-hipError_t launch_pass(int log2nf, int cols, bool first, bool swap_in, bool swap_out, const PassArgs &a,
+// workgroup (pass_cols_ok); layout: the LDS layout of the stage exchanges (kPadded or kHalves).
+hipError_t launch_pass(int log2nf, int cols, int layout, bool first, bool swap_in, bool swap_out, const PassArgs &a,
                        uint64_t batch, hipStream_t stream);
 
 // ---- ragged batches: one persistent launch over items of mixed sizes -------------------------

@@ -110,8 +110,8 @@ size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 }
 
-// Pass split of a multi-pass size: log2 N_1 .. log2 N_p (N_1 is transformed last), and the
-// columns per workgroup of each factor's pass. The environment variable ONEFFT_PASS_SPLIT
+// Pass split of a multi-pass size: log2 N_1 .. log2 N_p (N_1 is transformed last), the columns
+// per workgroup of each factor's pass, and the passes' LDS layout. The environment variable ONEFFT_PASS_SPLIT
 // ("a,b[,c[,d]]", log2 N_1 first) overrides for sizes whose factors it matches, to sweep splits
 // without rebuilding (the reference's analogue is its ONEFFT_X86_TUNING build switch,
 // src/onefft.c:652-697); else the tuning table's entry; else an even split into passes of at
@@ -119,12 +119,18 @@ size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 int choose_cols(int log2nf);
 int choose_split_factors(unsigned log2n, int regime, int *factors);
 
-int choose_split(unsigned log2n, int regime, int *factors, int *cols) {
+int choose_split(unsigned log2n, int regime, int *factors, int *cols, int *layout) {
     const int p = choose_split_factors(log2n, regime, factors);
     const onefft::TuneEntry &t = onefft::tune_lookup(log2n, regime);
     bool from_table = t.kind == onefft::TuneEntry::kMulti && t.passes == p;
     for (int i = 0; i < p && from_table; i++) from_table = t.split[i] == factors[i];
     for (int i = 0; i < p; i++) cols[i] = from_table ? t.cols[i] : choose_cols(factors[i]);
+    *layout = from_table ? t.pass_layout : onefft::kPadded;
+    // ONEFFT_PASS_LAYOUT: 0 (padded) or 2 (halves) for every pass, for measurement
+    if (const char *env = std::getenv("ONEFFT_PASS_LAYOUT")) {
+        const int v = std::atoi(env);
+        if (v == onefft::kPadded || v == onefft::kHalves) *layout = v;
+    }
     // ONEFFT_PASS_COLS: one count for every pass, or a list "c_1,..,c_p" (N_1 first); counts a
     // pass does not allow are ignored
     if (const char *env = std::getenv("ONEFFT_PASS_COLS")) {
@@ -208,6 +214,7 @@ struct PassPlan {
 // The passes of a multi-pass transform, in execution order.
 struct PassConfig {
     int passes = 0;
+    int layout = onefft::kPadded;  // LDS layout of every pass
     PassPlan pass[onefft::kMaxPasses];
 };
 
@@ -280,7 +287,7 @@ int execute_device(const onefft_plan *plan, const cf *d_in, cf *d_out, uint64_t 
             a.out = d_out + done * out_dist;
             a.in_dist = first ? in_dist : out_dist;
             a.out_dist = out_dist;
-            hipError_t e = launch_pass(cfg.pass[i].log2nf, cfg.pass[i].cols, first, first && inverse,
+            hipError_t e = launch_pass(cfg.pass[i].log2nf, cfg.pass[i].cols, cfg.layout, first, first && inverse,
                                        last && inverse, a, chunk, stream);
             if (e != hipSuccess) {
                 set_error("pass launch", e);
@@ -341,10 +348,11 @@ extern "C" onefft_plan *onefft_plan_create(size_t n) {
             for (size_t i = 0; i < hi_count; i++) root64((uint64_t)i << lo_bits, log2n, &hi[i].r, &hi[i].i);
         }
         for (int r = 0; r < 2; r++) {
-            int factors[kMaxPasses], cols[kMaxPasses];
-            const int p = choose_split(log2n, r, factors, cols);
+            int factors[kMaxPasses], cols[kMaxPasses], layout = kPadded;
+            const int p = choose_split(log2n, r, factors, cols, &layout);
             PassConfig &cfg = plan->cfg[r];
             cfg.passes = p;
+            cfg.layout = layout;
             // execution order: factor f = p, p-1, ..., 1 (factors[f-1])
             for (int i = 0; i < p; i++) {
                 const int f = p - i;  // 1-based factor index

@@ -53,6 +53,7 @@ struct TuneEntry {
     int passes = 0;              // kMulti: the split, log2 N_1 first, and columns per pass factor
     int split[kMaxPasses] = {};
     int cols[kMaxPasses] = {};
+    int pass_layout = 0;         // kMulti: LDS layout of every pass (kPadded or kHalves)
 };
 const TuneEntry &tune_lookup(unsigned log2n, int regime);
 

@@ -2,8 +2,9 @@
 // multi-pass split and columns per workgroup that measured fastest on this GPU.
 //
 // File: tab-separated lines "log2n  kind  params  gb_s  batch", '#' comments, where kind is
-// "single" (params "nt=0|1,layout=0..2[,tw64=0..3]") or "multi" (params "split=a,b[,c[,d]];cols=x,y[,..]",
-// log2 N_1 first). The batch the line was measured at sets its working-set regime
+// "single" (params "nt=0|1,layout=0..2[,tw64=0..3]") or "multi" (params
+// "split=a,b[,c[,d]];cols=x,y[,..][;layout=0|2]", log2 N_1 first; layout: the passes' LDS layout,
+// padded when absent). The batch the line was measured at sets its working-set regime
 // (runtime.hpp: tune_regime); a size may have a line per regime. Written by `python -m onefft_amd.tune`; read once, at the first plan creation,
 // from $ONEFFT_TUNE_FILE or tune_gfx950.tsv next to the library. Plan creation stays O(n) with no
 // measuring, as with the reference's planner (its tuning is compiled in, src/onefft.c:652-697).
@@ -80,6 +81,11 @@ void parse_line(const char *line) {
             sum += e.split[i];
         }
         if (sum != log2n) return;
+        if (const char *lp = std::strstr(params, "layout=")) {
+            const int layout = std::atoi(lp + 7);
+            if (layout != kPadded && layout != kHalves) return;
+            e.pass_layout = layout;
+        }
         e.kind = TuneEntry::kMulti;
     } else {
         return;

@@ -9,7 +9,7 @@ tuning table the library reads at plan creation (tuning.hip).
   level);
 * larger sizes: every split n = N_1 ... N_p into 2..4 passes with factors 2^6..2^10 (4 passes only
   from 2^24; two- and three-pass splits also with 2^11 and 2^12), each with the default and the
-  widest allowed columns per workgroup;
+  widest allowed columns per workgroup and with the padded or the halves LDS layout;
 each in two working-set regimes, because the best choice differs between them: a batch whose
 input plus output fit the 256 MiB Infinity Cache (16 n batch = 256 MiB up to 2^20 points; one
 transform from 2^21 points, where a single large transform is the common use and has too few tiles
@@ -18,7 +18,7 @@ per pass for the batch's best split), and a batch of 1 GiB in and 1 GiB out (one
 tune_regime).
 
 Each candidate is a plan created under the environment overrides (ONEFFT_BATCH_VARIANT,
-ONEFFT_PASS_SPLIT, ONEFFT_PASS_COLS) and timed with HIP events: after 3 warm-up launches, the
+ONEFFT_PASS_SPLIT, ONEFFT_PASS_COLS, ONEFFT_PASS_LAYOUT) and timed with HIP events: after 3 warm-up launches, the
 fastest of 5 samples of 10 back-to-back launches (single launches timed alone varied by more than
 the 2-5 % that separates the best variants). All candidates of a size give results within FP32 rounding of each other; the
 tuner checks each against the first candidate (rel-L2 <= 3e-7 log2 n). The fastest is written.
@@ -108,9 +108,12 @@ def candidates(L: int, single: bool):
             if tuple(cols) in seen:
                 continue
             seen.add(tuple(cols))
-            out.append(dict(kind="multi", params=f"split={','.join(map(str, s))};cols={','.join(map(str, cols))}",
-                            env={"ONEFFT_PASS_SPLIT": ",".join(map(str, s)),
-                                 "ONEFFT_PASS_COLS": ",".join(map(str, cols))}))
+            for layout in (0, 2):  # the passes' LDS layout: padded, or halves (kernels.hpp)
+                params = f"split={','.join(map(str, s))};cols={','.join(map(str, cols))}"
+                out.append(dict(kind="multi", params=params + (";layout=2" if layout else ""),
+                                env={"ONEFFT_PASS_SPLIT": ",".join(map(str, s)),
+                                     "ONEFFT_PASS_COLS": ",".join(map(str, cols)),
+                                     "ONEFFT_PASS_LAYOUT": str(layout)}))
     return out
 
 
@@ -149,7 +152,7 @@ def main(argv=None):
     ap.add_argument("--out", default=DEFAULT_OUT)
     args = ap.parse_args(argv)
     os.environ["ONEFFT_TUNE_FILE"] = os.path.join(ROOT, "no-tuning-table-while-tuning")
-    for k in ("ONEFFT_BATCH_VARIANT", "ONEFFT_PASS_SPLIT", "ONEFFT_PASS_COLS"):
+    for k in ("ONEFFT_BATCH_VARIANT", "ONEFFT_PASS_SPLIT", "ONEFFT_PASS_COLS", "ONEFFT_PASS_LAYOUT"):
         os.environ.pop(k, None)
 
     import torch

@@ -35,7 +35,7 @@ def run_with_table(tmp_path, text):
     table = tmp_path / "table.tsv"
     table.write_text(text)
     env = dict(os.environ, ONEFFT_TUNE_FILE=str(table))
-    for k in ("ONEFFT_BATCH_VARIANT", "ONEFFT_PASS_SPLIT", "ONEFFT_PASS_COLS"):
+    for k in ("ONEFFT_BATCH_VARIANT", "ONEFFT_PASS_SPLIT", "ONEFFT_PASS_COLS", "ONEFFT_PASS_LAYOUT"):
         env.pop(k, None)
     r = subprocess.run([sys.executable, "-c", CHILD.format(root=ROOT)], env=env, capture_output=True, text=True,
                        timeout=300)
@@ -49,11 +49,19 @@ def test_table_choices_are_used_and_correct(tmp_path):
                              "22\tmulti\tsplit=6,6,10;cols=32,16,8\t1\t1\n")
 
 
+def test_table_pass_layout_is_used_and_correct(tmp_path):
+    run_with_table(tmp_path, "# test table\n"
+                             "12\tsingle\tnt=1,layout=2,tw64=2\t1\t1\n"
+                             "16\tmulti\tsplit=6,10;cols=64,8;layout=2\t1\t1\n"
+                             "22\tmulti\tsplit=11,11;cols=4,4;layout=2\t1\t1\n")
+
+
 def test_malformed_table_is_ignored(tmp_path):
     run_with_table(tmp_path, "12 single garbage\n"
                              "16\tmulti\tsplit=6,9;cols=16,16\t1\t1\n"      # factors do not sum to 16
                              "22\tmulti\tsplit=11,11;cols=8,8\t1\t1\n"      # 8 columns of 2^11 points
                              "22\tmulti\tsplit=13,9;cols=2,16\t1\t1\n"      # factor above 2^12
+                             "16\tmulti\tsplit=8,8;cols=16,16;layout=1\t1\t1\n"  # no swizzled passes
                              "99\tsingle\tnt=1,layout=0\t1\t1\n")
 
 
@@ -108,7 +116,8 @@ def test_every_single_pass_variant(monkeypatch):
                                               ("12,11", "2,4", 1), ("12,12", "2,2", 1), ("6,12", "16,2", 2),
                                               ("6,6,11", "16,16,4", 1), ("6,12,6", "16,2,16", 2),
                                               ("11,6,6", "4,16,16", 1)])
-def test_splits_with_wide_factors(monkeypatch, split, cols, batch):
+@pytest.mark.parametrize("layout", [0, 2])
+def test_splits_with_wide_factors(monkeypatch, split, cols, batch, layout):
     """The 2^11- and 2^12-point pass factors (4 and 2 columns per tile) that two- and three-pass
     splits may use, chosen at plan creation through ONEFFT_PASS_SPLIT / ONEFFT_PASS_COLS: every
     transform of a small batch, forward, against the FP64 oracle, and the round trip."""
@@ -121,12 +130,14 @@ def test_splits_with_wide_factors(monkeypatch, split, cols, batch):
 
     L = sum(int(v) for v in split.split(","))
     n = 1 << L
-    print(f"two-pass split {split} cols {cols} n=2^{L} x{batch}", flush=True)
+    print(f"split {split} cols {cols} layout {layout} n=2^{L} x{batch}", flush=True)
     monkeypatch.setenv("ONEFFT_PASS_SPLIT", split)
     monkeypatch.setenv("ONEFFT_PASS_COLS", cols)
+    monkeypatch.setenv("ONEFFT_PASS_LAYOUT", str(layout))
     p = F.Plan(n)
     monkeypatch.delenv("ONEFFT_PASS_SPLIT")
     monkeypatch.delenv("ONEFFT_PASS_COLS")
+    monkeypatch.delenv("ONEFFT_PASS_LAYOUT")
     s = torch.cuda.current_stream().cuda_stream
     x = O.normal(70 + L, n * batch)
     d = torch.from_numpy(x).cuda()

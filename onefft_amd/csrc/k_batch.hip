@@ -31,8 +31,8 @@ static hipError_t launch_small(bool inverse, bool nt, const BatchArgs &a, hipStr
 
 template <int LOG2N>
 static hipError_t launch_batch_n(bool inverse, const BatchArgs &a, BatchVariant v, hipStream_t stream) {
-    if constexpr (LOG2N <= 4) {
-        // contiguous batches of up to 16 points: coalesced blocks (k_batch.hpp: fft_small_kernel)
+    if constexpr (LOG2N <= 5) {
+        // contiguous batches of up to 32 points: coalesced blocks (k_batch.hpp: fft_small_kernel)
         if (a.batch > 0 && a.in_dist == (1u << LOG2N) && a.out_dist == (1u << LOG2N) && a.batch <= (~0ull >> 5))
             return launch_small<LOG2N>(inverse, v.nt, a, stream);
     }

@@ -48,21 +48,24 @@ inline void launch_variant(bool inverse, const BatchArgs &a, dim3 grid, hipStrea
                            a.in, a.out, a.tw, a.tw64, a.in_dist, a.out_dist, a.batch);
 }
 
-// Contiguous batches of transforms of n <= 16 points (in_dist = out_dist = n; the reference's small
+// Contiguous batches of transforms of n <= 32 points (in_dist = out_dist = n; the reference's small
 // kernels, src/onefft_avx512.inc n32/n64/n128 and its 1, 2 and 4-point closed forms,
 // src/onefft.c:158-181). Each 256-thread workgroup moves one block of 4096 values with coalesced
 // 8-byte accesses (thread t: values t + 256 r), exchanges them through LDS so that each thread
 // holds 16 / n whole transforms, runs the register DFTs, and exchanges back before the coalesced
-// stores. (The general kernel gives each thread one transform of n points: a wavefront's accesses
-// then touch 64 lines 8 n bytes apart.) Only the last workgroup's block can be partial; its lanes
-// skip the values past the end.
+// stores. For n = 32 two threads share a transform: thread j of it takes the 16-point DFT of
+// x[2k + j] (the odd half then times w_32^m), and after one more exchange each combines 8 of the
+// output pairs X[m], X[m + 16] = E[m] +- O[m], at the LDS positions it read them from. (The
+// general kernel gives each thread 16 points of one transform: a wavefront's accesses then touch
+// lines 8 n bytes apart.) Only the last workgroup's block can be partial; its lanes skip the
+// values past the end.
 constexpr uint32_t kSmallBlock = 4096;
 
 template <int LOG2N, bool INV, bool NT>
 __global__ __launch_bounds__(256) void fft_small_kernel(const cf *__restrict__ in, cf *__restrict__ out,
                                                          uint64_t total) {
     constexpr int N = 1 << LOG2N;
-    static_assert(LOG2N <= 4, "up to 16 points: one register DFT per 16 / n values");
+    static_assert(LOG2N <= 5, "up to 32 points: one register DFT per 16 / n values, or two threads per 32");
     __shared__ cf lds[LOG2N > 0 ? lds_span(kSmallBlock) : 1];
     const uint32_t t = threadIdx.x;
     const uint64_t base = (uint64_t)blockIdx.x * kSmallBlock;
@@ -78,7 +81,27 @@ __global__ __launch_bounds__(256) void fft_small_kernel(const cf *__restrict__ i
             v[r] = t + c < count ? io.load(t, c) : mk(0.f, 0.f);
         });
     }
-    if constexpr (LOG2N > 0) {
+    if constexpr (LOG2N == 5) {
+        static_for<0, 16>([&](auto r) { lds[lds_pad(t + decltype(r)::value * 256u)] = v[r]; });
+        __syncthreads();
+        // thread t: half j of transform c, whose values start at 32 c
+        const uint32_t c32 = (t >> 1) * 32u, j = t & 1u;
+        static_for<0, 16>([&](auto k) { v[k] = maybe_conj<INV>(lds[lds_pad(c32 + 2u * decltype(k)::value + j)]); });
+        dft<16>(v);
+        if (j) static_for<1, 16>([&](auto m) { v[m] = twc<32, decltype(m)::value>(v[m]); });
+        __syncthreads();  // every thread has read its inputs
+        static_for<0, 16>([&](auto m) { lds[lds_pad(c32 + 16u * j + decltype(m)::value)] = v[m]; });
+        __syncthreads();
+        // outputs m and m + 16 for m = 8 j .. 8 j + 7, from E[m] (position m) and O[m] (m + 16)
+        static_for<0, 8>([&](auto i) {
+            const uint32_t m = 8u * j + decltype(i)::value;
+            const cf e = lds[lds_pad(c32 + m)], o = lds[lds_pad(c32 + 16u + m)];
+            lds[lds_pad(c32 + m)] = maybe_conj<INV>(add(e, o));
+            lds[lds_pad(c32 + 16u + m)] = maybe_conj<INV>(sub(e, o));
+        });
+        __syncthreads();
+        static_for<0, 16>([&](auto r) { v[r] = lds[lds_pad(t + decltype(r)::value * 256u)]; });
+    } else if constexpr (LOG2N > 0) {
         static_for<0, 16>([&](auto r) { lds[lds_pad(t + decltype(r)::value * 256u)] = v[r]; });
         __syncthreads();
         // thread t now works on values 16 t .. 16 t + 15 alone: 16 / n whole transforms

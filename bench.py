@@ -78,7 +78,8 @@ def variants_ab(torch, F, sizes):
 
     variants = {"padded": "0,0,0", "padded_nt": "1,0,0", "swizzled": "0,1,0", "swizzled_nt": "1,1,0",
                 "padded_tw64": "0,0,1", "padded_nt_tw64": "1,0,1", "padded_nt_tw64_ns16": "1,0,2",
-                "padded_nt_tw64_all": "1,0,3"}
+                "padded_nt_tw64_all": "1,0,3", "halves_nt": "1,2,0", "halves_nt_tw64_ns16": "1,2,2",
+                "halves_nt_tw64_all": "1,2,3"}
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
     old = os.environ.get("ONEFFT_BATCH_VARIANT")

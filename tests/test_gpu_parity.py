@@ -224,7 +224,7 @@ def test_device_calls_reject_overlap():
 
 # ---- batches ---------------------------------------------------------------------------------------
 
-@pytest.mark.parametrize("L,batch", [(0, 5), (0, 70001), (1, 3), (2, 100003), (3, 257), (4, 1000), (4, 65537), (5, 129),
+@pytest.mark.parametrize("L,batch", [(0, 5), (0, 70001), (1, 3), (2, 100003), (3, 257), (4, 1000), (4, 65537), (5, 129), (5, 70001), (6, 3), (6, 65601),
                                      (8, 77), (10, 33), (11, 5),
                                      (12, 9), (13, 3), (14, 3), (15, 3), (16, 2), (18, 2)])
 def test_batch(L, batch):

@@ -357,6 +357,42 @@ def test_threads_bitwise():
     lib.onefft_cache_clear()
 
 
+def test_threads_staged_ring_bitwise():
+    """Host calls large enough for the staging ring (several 4 MiB chunks, copied by the shared
+    copy pool) from several threads at once, at two sizes: every result is bit-identical to the
+    same call made alone."""
+    lib = F.lib()
+    sizes = (1 << 20, 1 << 22)
+    data = {n: O.xorshift(2 * n, 77 + n.bit_length()) for n in sizes}
+    want = {}
+    for n in sizes:
+        fwd, inv = np.empty_like(data[n]), np.empty_like(data[n])
+        assert lib.onefft_forward(data[n].ctypes.data, fwd.ctypes.data, n) == 0
+        assert lib.onefft_inverse(data[n].ctypes.data, inv.ctypes.data, n) == 0
+        want[n] = (fwd, inv)
+    mismatches = []
+
+    def work(t):
+        n = sizes[t & 1]
+        x = data[n]
+        out = np.empty_like(x)
+        bad = 0
+        for r in range(6):
+            inverse = (r + t) & 1
+            fn = lib.onefft_inverse if inverse else lib.onefft_forward
+            bad += fn(x.ctypes.data, out.ctypes.data, n) != 0
+            bad += not np.array_equal(out, want[n][inverse])
+        mismatches.append(bad)
+
+    threads = [threading.Thread(target=work, args=(t,)) for t in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert mismatches == [0, 0, 0, 0]
+    lib.onefft_cache_clear()
+
+
 # ---- the reference's own test programs against this library ------------------------------------------
 
 @pytest.mark.parametrize("prog,args", [("ref_test_onefft", ["18"]), ("ref_test_cpp", [])])

@@ -41,7 +41,7 @@ __global__ __launch_bounds__(pass_wg(LOG2NF, C)) void fft_pass_kernel(PassArgs a
 template <int LOG2NF, int C, int LAYOUT>
 inline hipError_t launch_pass_nc(bool first, bool swap_in, bool swap_out, const PassArgs &a, uint64_t batch,
                                  hipStream_t stream) {
-    if constexpr (!pass_cols_ok(LOG2NF, C)) {
+    if constexpr (!pass_cols_ok(LOG2NF, C, LAYOUT)) {
         return hipErrorInvalidValue;
     } else {
         if (a.columns % C || batch == 0 || batch > 65535) return hipErrorInvalidValue;

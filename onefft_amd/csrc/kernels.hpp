@@ -56,13 +56,16 @@ constexpr int kPassMaxLog2 = 12;  // largest sub-transform of a pass
 
 // Points per thread of the pass kernel for a 2^log2nf sub-transform, and the columns per
 // workgroup: C adjacent columns make every global access a run of C complex values (C * 8
-// bytes). The default C and the choices a plan may make (LDS of C columns <= 72 KiB). The
-// 2^11- and 2^12-point sub-transforms fit only 4 and 2 columns (32- and 16-byte runs): they exist
-// for two-pass splits of sizes whose data stays in the MALL, where short runs cost less.
+// bytes). The default C and the choices a plan may make: tiles of at most 8192 points with the
+// padded layout (LDS of C columns <= 72 KiB), 16384 with the halves layout, whose tiles need half
+// the LDS. The 2^11- and 2^12-point sub-transforms fit only 4 and 2 columns with the padded layout
+// (32- and 16-byte runs): they exist for two-pass splits of sizes whose data stays in the MALL,
+// where short runs cost less.
 constexpr int pass_log2p(int log2nf) { return log2nf < 4 ? log2nf : 4; }
-constexpr bool pass_cols_ok(int log2nf, int cols) {
+constexpr int pass_tile_max(int layout) { return layout == 2 /* fft_common.hpp: kHalves */ ? 16384 : 8192; }
+constexpr bool pass_cols_ok(int log2nf, int cols, int layout = 0) {
     return (cols == 2 || cols == 4 || cols == 8 || cols == 16 || cols == 32 || cols == 64) &&
-           (cols << log2nf) <= 8192 && (cols << (log2nf - pass_log2p(log2nf))) <= 1024 &&
+           (cols << log2nf) <= pass_tile_max(layout) && (cols << (log2nf - pass_log2p(log2nf))) <= 1024 &&
            (cols << (log2nf - pass_log2p(log2nf))) >= 64 && (cols >= 8 || log2nf >= 11);
 }
 // default: 16 columns (128-byte runs) where that is allowed, else the nearest allowed count

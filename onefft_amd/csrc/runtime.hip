@@ -144,9 +144,12 @@ int choose_split(unsigned log2n, int regime, int *factors, int *cols, int *layou
         }
         for (int i = 0; i < p; i++) {
             const int want = k == p ? c[i] : c[0];
-            if (k && onefft::pass_cols_ok(factors[i], want)) cols[i] = want;
+            if (k && onefft::pass_cols_ok(factors[i], want, *layout)) cols[i] = want;
         }
     }
+    // a table's counts for the halves layout may not fit the padded one that ONEFFT_PASS_LAYOUT chose
+    for (int i = 0; i < p; i++)
+        if (!onefft::pass_cols_ok(factors[i], cols[i], *layout)) cols[i] = choose_cols(factors[i]);
     return p;
 }
 

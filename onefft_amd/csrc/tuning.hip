@@ -74,18 +74,18 @@ void parse_line(const char *line) {
         e.passes = parse_list(sp + 6, e.split, kMaxPasses);
         const int nc = parse_list(cp + 5, e.cols, kMaxPasses);
         if (e.passes < 2 || nc != e.passes) return;
-        int sum = 0;
-        for (int i = 0; i < e.passes; i++) {
-            if (e.split[i] < kPassMinLog2 || e.split[i] > kPassMaxLog2) return;
-            if (!pass_cols_ok(e.split[i], e.cols[i])) return;
-            sum += e.split[i];
-        }
-        if (sum != log2n) return;
         if (const char *lp = std::strstr(params, "layout=")) {
             const int layout = std::atoi(lp + 7);
             if (layout != kPadded && layout != kHalves) return;
             e.pass_layout = layout;
         }
+        int sum = 0;
+        for (int i = 0; i < e.passes; i++) {
+            if (e.split[i] < kPassMinLog2 || e.split[i] > kPassMaxLog2) return;
+            if (!pass_cols_ok(e.split[i], e.cols[i], e.pass_layout)) return;
+            sum += e.split[i];
+        }
+        if (sum != log2n) return;
         e.kind = TuneEntry::kMulti;
     } else {
         return;

@@ -9,7 +9,8 @@ tuning table the library reads at plan creation (tuning.hip).
   level);
 * larger sizes: every split n = N_1 ... N_p into 2..4 passes with factors 2^6..2^10 (4 passes only
   from 2^24; two- and three-pass splits also with 2^11 and 2^12), each with the default and the
-  widest allowed columns per workgroup and with the padded or the halves LDS layout;
+  widest allowed columns per workgroup and with the padded or the halves LDS layout, and with the
+  halves layout's 16384-point tiles for the input pass;
 each in two working-set regimes, because the best choice differs between them: a batch whose
 input plus output fit the 256 MiB Infinity Cache (16 n batch = 256 MiB up to 2^20 points; one
 transform from 2^21 points, where a single large transform is the common use and has too few tiles
@@ -42,11 +43,13 @@ HALVES_MIN_LOG2 = 8  # kernels.hpp: kHalvesMinLog2
 GROUP = 10  # launches per timed sample: back to back, as batches run, so a 2 % difference shows
 
 
-def pass_cols_ok(log2nf: int, cols: int) -> bool:
-    """The kernels' rule (kernels.hpp: pass_cols_ok)."""
+def pass_cols_ok(log2nf: int, cols: int, layout: int = 0) -> bool:
+    """The kernels' rule (kernels.hpp: pass_cols_ok): tiles of at most 8192 points with the padded
+    layout, 16384 with the halves layout."""
     lp = min(log2nf, 4)
     threads = cols << (log2nf - lp)
-    return (cols in (2, 4, 8, 16, 32, 64) and (cols << log2nf) <= 8192 and 64 <= threads <= 1024
+    tile_max = 16384 if layout == 2 else 8192
+    return (cols in (2, 4, 8, 16, 32, 64) and (cols << log2nf) <= tile_max and 64 <= threads <= 1024
             and (cols >= 8 or log2nf >= 11))
 
 
@@ -57,8 +60,8 @@ def default_cols(log2nf: int) -> int:
     return 2
 
 
-def widest_cols(log2nf: int) -> int:
-    return max(c for c in (2, 4, 8, 16, 32, 64) if pass_cols_ok(log2nf, c))
+def widest_cols(log2nf: int, layout: int = 0) -> int:
+    return max(c for c in (2, 4, 8, 16, 32, 64) if pass_cols_ok(log2nf, c, layout))
 
 
 def tw64_levels(log2n: int):
@@ -103,17 +106,24 @@ def candidates(L: int, single: bool):
                      env={"ONEFFT_BATCH_VARIANT": f"{nt},{lay},{t64}"}) for nt, lay, t64 in combos]
     out = []
     for s in splits(L):
+        # the passes' LDS layout (padded, or halves: kernels.hpp), with the default and the widest
+        # columns; the halves layout also with its wider tiles for the first (input) pass, whose
+        # 2^11- and 2^12-point columns gain 2x longer runs (its in-place passes keep fewer
+        # resident workgroups with the wider tiles)
+        cands = []
+        for layout in (0, 2):
+            cands += [(layout, [default_cols(f) for f in s]), (layout, [widest_cols(f) for f in s])]
+        cands.append((2, [widest_cols(f) for f in s[:-1]] + [widest_cols(s[-1], 2)]))
         seen = set()
-        for cols in ([default_cols(f) for f in s], [widest_cols(f) for f in s]):
-            if tuple(cols) in seen:
+        for layout, cols in cands:
+            if (layout, tuple(cols)) in seen:
                 continue
-            seen.add(tuple(cols))
-            for layout in (0, 2):  # the passes' LDS layout: padded, or halves (kernels.hpp)
-                params = f"split={','.join(map(str, s))};cols={','.join(map(str, cols))}"
-                out.append(dict(kind="multi", params=params + (";layout=2" if layout else ""),
-                                env={"ONEFFT_PASS_SPLIT": ",".join(map(str, s)),
-                                     "ONEFFT_PASS_COLS": ",".join(map(str, cols)),
-                                     "ONEFFT_PASS_LAYOUT": str(layout)}))
+            seen.add((layout, tuple(cols)))
+            params = f"split={','.join(map(str, s))};cols={','.join(map(str, cols))}"
+            out.append(dict(kind="multi", params=params + (";layout=2" if layout else ""),
+                            env={"ONEFFT_PASS_SPLIT": ",".join(map(str, s)),
+                                 "ONEFFT_PASS_COLS": ",".join(map(str, cols)),
+                                 "ONEFFT_PASS_LAYOUT": str(layout)}))
     return out
 
 

@@ -56,12 +56,19 @@ def test_table_pass_layout_is_used_and_correct(tmp_path):
                              "22\tmulti\tsplit=11,11;cols=4,4;layout=2\t1\t1\n")
 
 
+def test_table_wide_halves_tiles_are_used_and_correct(tmp_path):
+    run_with_table(tmp_path, "# test table\n"
+                             "16\tmulti\tsplit=6,10;cols=64,16;layout=2\t1\t1\n"
+                             "22\tmulti\tsplit=11,11;cols=8,8;layout=2\t1\t1\n")
+
+
 def test_malformed_table_is_ignored(tmp_path):
     run_with_table(tmp_path, "12 single garbage\n"
                              "16\tmulti\tsplit=6,9;cols=16,16\t1\t1\n"      # factors do not sum to 16
                              "22\tmulti\tsplit=11,11;cols=8,8\t1\t1\n"      # 8 columns of 2^11 points
                              "22\tmulti\tsplit=13,9;cols=2,16\t1\t1\n"      # factor above 2^12
                              "16\tmulti\tsplit=8,8;cols=16,16;layout=1\t1\t1\n"  # no swizzled passes
+                             "22\tmulti\tsplit=11,11;cols=16,16;layout=2\t1\t1\n"  # 32768-point tiles
                              "99\tsingle\tnt=1,layout=0\t1\t1\n")
 
 
@@ -121,13 +128,31 @@ def test_splits_with_wide_factors(monkeypatch, split, cols, batch, layout):
     """The 2^11- and 2^12-point pass factors (4 and 2 columns per tile) that two- and three-pass
     splits may use, chosen at plan creation through ONEFFT_PASS_SPLIT / ONEFFT_PASS_COLS: every
     transform of a small batch, forward, against the FP64 oracle, and the round trip."""
+    check_split(monkeypatch, split, cols, batch, layout)
+
+
+@pytest.mark.parametrize("split,cols,batch", [("10,11", "16,8", 3), ("11,11", "8,8", 2), ("12,12", "4,4", 1),
+                                              ("9,9", "32,32", 4), ("8,10", "64,16", 4),
+                                              ("6,12,6", "16,4,16", 2), ("11,6,6", "8,16,16", 1),
+                                              ("8,11,6", "64,8,16", 1)])
+def test_halves_wide_tiles(monkeypatch, split, cols, batch):
+    """The halves layout's 16384-point pass tiles (1024 threads: 2^8 x 64, 2^9 x 32, 2^10 x 16,
+    2^11 x 8, 2^12 x 4 columns), as the input pass and as in-place passes."""
+    check_split(monkeypatch, split, cols, batch, 2)
+
+
+def check_split(monkeypatch, split, cols, batch, layout):
     import numpy as np
     import torch
 
     import onefft_amd as F
     from conftest import tol_fp64
     from oracle import oracle as O
+    from onefft_amd.tune import pass_cols_ok
 
+    # counts a pass does not allow would be ignored by the library: every case must be a real one
+    for f, c in zip(split.split(","), cols.split(",")):
+        assert pass_cols_ok(int(f), int(c), layout), (split, cols, layout)
     L = sum(int(v) for v in split.split(","))
     n = 1 << L
     print(f"split {split} cols {cols} layout {layout} n=2^{L} x{batch}", flush=True)

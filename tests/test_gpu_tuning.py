@@ -83,6 +83,49 @@ def test_committed_table_parses():
         assert 0 <= int(log2n) <= 30 and kind in ("single", "multi") and float(gbs) > 0 and int(batch) >= 1
 
 
+def test_committed_table_multi_choices():
+    """Every multi-pass line of the committed table up to 2^26 points, run at the batch it was
+    tuned at (so the library takes that line's regime and choice): the first and the last
+    transform of the batch against the FP64 oracle."""
+    import numpy as np
+    import torch
+
+    import onefft_amd as F
+    from conftest import tol_fp64
+    from oracle import oracle as O
+
+    path = os.path.join(ROOT, "onefft_amd", "tune_gfx950.tsv")
+    if not os.path.exists(path):
+        pytest.skip("no committed tuning table")
+    s = torch.cuda.current_stream().cuda_stream
+    checked = 0
+    for line in open(path):
+        if line.startswith("#") or not line.strip():
+            continue
+        log2n, kind, params, _, batch = line.rstrip("\n").split("\t")
+        L, batch = int(log2n), int(batch)
+        if kind != "multi" or L > 26:
+            continue
+        n = 1 << L
+        print(f"2^{L} x{batch}: {params}", flush=True)
+        g = torch.Generator(device="cuda").manual_seed(1000 + L)
+        x = torch.randn(batch, n, dtype=torch.complex64, device="cuda", generator=g)
+        y = torch.empty_like(x)
+        p = F.Plan(n)
+        p.batch_ptr(x.data_ptr(), y.data_ptr(), batch, n, n, False, s)
+        torch.cuda.synchronize()
+        for b in sorted({0, batch - 1}):
+            xin = x[b].cpu().numpy().view(np.float32)
+            got = y[b].cpu().numpy().view(np.float32)
+            e = O.rel_rms(got, O.fft_f64(xin, n))
+            assert e < tol_fp64(L), (L, batch, params, b, e)
+        p.close()
+        del x, y
+        torch.cuda.empty_cache()
+        checked += 1
+    assert checked >= 20
+
+
 def test_every_single_pass_variant(monkeypatch):
     """Every kernel variant the tuner may choose, at every single-pass size, both directions,
     against the FP64 oracle (variants are chosen at plan creation from ONEFFT_BATCH_VARIANT)."""

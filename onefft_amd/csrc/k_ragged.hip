@@ -55,59 +55,63 @@ struct Column {
 
 __device__ __forceinline__ cf swap_if(cf v, bool s) { return s ? swap_ri(v) : v; }
 
-template <int R0>
-__device__ __forceinline__ void first_stage(const Column &io, uint32_t N, uint32_t P, uint32_t j, bool only, cf *L,
+// The first stage: BF = P / R0 butterflies of radix R0 per thread (BF = 1 when the column has at
+// most 16 points, else 16 / R0). All P loads are issued before any arithmetic, so a thread waits
+// for global memory once per column rather than once per butterfly.
+template <int R0, int BF>
+__device__ __forceinline__ void first_stage(const Column &io, uint32_t N, uint32_t j, bool only, cf *L,
                                             cf (&v)[16]) {
+    constexpr uint32_t P = R0 * BF;
     const uint32_t T = N / P, stride = N / R0;
-    const uint32_t BF = P / R0;  // 1..8, a run-time count: loops stay short
-    for (uint32_t q = 0; q < BF; q++) {
-        cf w[R0];
-        if (io.live) {
+    static_for<0, BF>([&](auto q) {
+        static_for<0, R0>([&](auto r) {
+            constexpr int qq = decltype(q)::value, rr = decltype(r)::value;
+            v[qq * R0 + rr] = io.live ? swap_if(io.src[(j + T * qq + rr * stride) * io.in_step], io.swap_in)
+                                      : mk(0.f, 0.f);
+        });
+    });
+    if (io.twiddle) {
+        const uint32_t mask = (1u << io.lo_bits) - 1;
+        auto W = [&](uint32_t x) {
+            x <<= io.shift;
+            return mul64(io.hi[x >> io.lo_bits], io.lo[x & mask]);
+        };
+        const cd step = W(stride * io.low);
+        static_for<0, BF>([&](auto q) {
+            constexpr int qq = decltype(q)::value;
+            cd tw = W((j + T * qq) * io.low);
             static_for<0, R0>([&](auto r) {
-                w[r] = swap_if(io.src[(j + T * q + decltype(r)::value * stride) * io.in_step], io.swap_in);
-            });
-        } else {
-            static_for<0, R0>([&](auto r) { w[r] = mk(0.f, 0.f); });
-        }
-        if (io.twiddle) {
-            const uint32_t mask = (1u << io.lo_bits) - 1;
-            auto W = [&](uint32_t x) {
-                x <<= io.shift;
-                return mul64(io.hi[x >> io.lo_bits], io.lo[x & mask]);
-            };
-            cd tw = W((j + T * q) * io.low);
-            const cd step = W(stride * io.low);
-            static_for<0, R0>([&](auto r) {
-                w[r] = mul(w[r], round32(tw));
+                v[qq * R0 + decltype(r)::value] = mul(v[qq * R0 + decltype(r)::value], round32(tw));
                 if constexpr (decltype(r)::value + 1 < R0) tw = mul64(tw, step);
             });
-        }
-        dft<R0>(w);
-        const uint32_t b = j + T * q;
+        });
+    }
+    static_for<0, BF>([&](auto q) {
+        constexpr int qq = decltype(q)::value;
+        dft<R0>(v + qq * R0);
+        const uint32_t b = j + T * qq;
         static_for<0, R0>([&](auto m) {
             constexpr int mm = decltype(m)::value;
             if (only) {
-                if (io.live) io.dst[(b + mm * stride) * io.out_step] = swap_if(w[mm], io.swap_out);
+                if (io.live) io.dst[(b + mm * stride) * io.out_step] = swap_if(v[qq * R0 + mm], io.swap_out);
             } else {
-                L[lds_pad(b * R0 + mm)] = w[mm];
+                L[lds_pad(b * R0 + mm)] = v[qq * R0 + mm];
             }
         });
-    }
-    (void)v;
+    });
 }
 
 __device__ void column_fft(const Column &io, uint32_t log2n, const cd *tw64, cf *L, uint32_t j) {
     const uint32_t N = 1u << log2n;
-    const uint32_t P = N < 16 ? N : 16;
     const uint32_t r0_log2 = log2n <= 4 ? log2n : (log2n % 4 ? log2n % 4 : 4);
-    const bool only = log2n <= 4;
+    const bool only = log2n <= 4;  // one stage: R0 = N points per thread
     cf v[16];
     switch (r0_log2) {
-    case 0: first_stage<1>(io, N, P, j, only, L, v); break;
-    case 1: first_stage<2>(io, N, P, j, only, L, v); break;
-    case 2: first_stage<4>(io, N, P, j, only, L, v); break;
-    case 3: first_stage<8>(io, N, P, j, only, L, v); break;
-    default: first_stage<16>(io, N, P, j, only, L, v); break;
+    case 0: first_stage<1, 1>(io, N, j, only, L, v); break;
+    case 1: only ? first_stage<2, 1>(io, N, j, only, L, v) : first_stage<2, 8>(io, N, j, only, L, v); break;
+    case 2: only ? first_stage<4, 1>(io, N, j, only, L, v) : first_stage<4, 4>(io, N, j, only, L, v); break;
+    case 3: only ? first_stage<8, 1>(io, N, j, only, L, v) : first_stage<8, 2>(io, N, j, only, L, v); break;
+    default: first_stage<16, 1>(io, N, j, only, L, v); break;
     }
     if (only) return;
     __syncthreads();

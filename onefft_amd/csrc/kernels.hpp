@@ -124,7 +124,7 @@ constexpr uint32_t kRaggedTaskPoints = 1u << kRaggedTaskLog2;
 constexpr int kRaggedSingleMaxLog2 = kRaggedTaskLog2;
 constexpr int kRaggedWG = kRaggedTaskPoints / 16;
 constexpr uint32_t kRaggedTileRun = 2;   // tiles per pass task (at most 256)
-constexpr int kRaggedSplitLog2 = 20;     // default split size (ONEFFT_RAGGED_SPLIT overrides)
+constexpr int kRaggedSplitLog2 = 19;     // default split size (ONEFFT_RAGGED_SPLIT overrides)
 
 enum RaggedKind : uint32_t { kRaggedSingle = 0, kRaggedFirst = 1, kRaggedSecond = 2, kRaggedWhole = 3 };
 

@@ -162,10 +162,18 @@ def extra_workloads(torch, F, stream):
         time_ms(torch, stream, fwd, reps)
         tf, ti = time_ms(torch, stream, fwd, reps), time_ms(torch, stream, inv, reps)
         rt = (torch.linalg.vector_norm(z / n - x) / torch.linalg.vector_norm(x)).item()
+        # context only: rocFFT through torch.fft.fft on the same input (plan built by the first call)
+        try:
+            vendor = lambda: torch.fft.fft(x)
+            time_ms(torch, stream, vendor, 3)
+            tv = round(time_ms(torch, stream, vendor, reps), 4)
+        except Exception:
+            tv = None
         large.append({"log2n": L, "fwd_ms": round(tf, 4), "inv_ms": round(ti, 4),
                       "fwd_gflops": round(5 * n * L / (tf * 1e-3) / 1e9, 1),
                       "fwd_alg_gb_s": round(16 * n / (tf * 1e-3) / 1e9, 1), "roundtrip_rel_l2": rt,
-                      "working_set": "MALL-resident (<= 256 MB)" if 16 * n <= (256 << 20) else "HBM"})
+                      "working_set": "MALL-resident (<= 256 MB)" if 16 * n <= (256 << 20) else "HBM",
+                      "rocfft_torch_fwd_ms": tv})
         pl.close()
         del x, y, z
     out["large"] = large

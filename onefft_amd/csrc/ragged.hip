@@ -153,13 +153,19 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
         const int v = std::atoi(env);
         if (v > kRaggedSingleMaxLog2 && v <= kRaggedMaxLog2 + 1) split = (uint32_t)v;
     }
+    // ONEFFT_RAGGED_TILE_RUN: tiles per pass task, 1..256 (for measurement)
+    uint32_t tile_run = kRaggedTileRun;
+    if (const char *env = std::getenv("ONEFFT_RAGGED_TILE_RUN")) {
+        const int v = std::atoi(env);
+        if (v >= 1 && v <= 256) tile_run = (uint32_t)v;
+    }
     auto pass_tasks = [&](uint32_t kind) {
         for (size_t k = 0; k < nbig; k++) {
             const onefft_ragged_item &it = items[order[k]];
             if (it.log2n < split) continue;
             const uint32_t tiles = 1u << (it.log2n - kRaggedTaskLog2);  // tiles per pass
-            for (uint32_t t0 = 0; t0 < tiles; t0 += kRaggedTileRun) {
-                const uint32_t run = tiles - t0 < kRaggedTileRun ? tiles - t0 : kRaggedTileRun;
+            for (uint32_t t0 = 0; t0 < tiles; t0 += tile_run) {
+                const uint32_t run = tiles - t0 < tile_run ? tiles - t0 : tile_run;
                 tasks.push_back({word(kind, it.log2n, it.inverse, run), (uint32_t)k, t0, 0});
             }
         }

@@ -85,6 +85,22 @@ def test_two_pass_items_many_tiles():
     check_items(items, x, y, extent)
 
 
+@pytest.mark.parametrize("split,tile_run", [("20", "1"), ("19", "3"), ("18", "256"), ("21", "2")])
+def test_split_and_tile_run_bitwise(monkeypatch, split, tile_run):
+    """Other split sizes and tile-run lengths (ONEFFT_RAGGED_SPLIT / ONEFFT_RAGGED_TILE_RUN) change
+    only which workgroup computes what: the results equal the defaults' bit for bit."""
+    sizes = [20, 19, 18, 17, 14, 13, 10, 4] * 2
+    items, extent = layout(sizes, gap=1, seed=9)
+    x = O.normal(34, extent)
+    want, _ = run(items, extent, x)
+    monkeypatch.setenv("ONEFFT_RAGGED_SPLIT", split)
+    monkeypatch.setenv("ONEFFT_RAGGED_TILE_RUN", tile_run)
+    got, _ = run(items, extent, x)
+    monkeypatch.delenv("ONEFFT_RAGGED_SPLIT")
+    monkeypatch.delenv("ONEFFT_RAGGED_TILE_RUN")
+    assert np.array_equal(got, want)
+
+
 def test_rejects_bad_items():
     lib = F.lib()
     for items in ([(0, 21, 0)], [(0, 4, 2)], [(0, 4, 0), (15, 4, 0)], [(16, 4, 0), (0, 5, 0)]):

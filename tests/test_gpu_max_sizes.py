@@ -62,25 +62,36 @@ def test_tone_roundtrip_parseval(L):
     torch.cuda.empty_cache()
 
 
+@pytest.mark.parametrize("wide", [False, True])
 @pytest.mark.parametrize("inverse", [False, True])
-def test_multipass_batch_across_launch_chunks(inverse):
+def test_multipass_batch_across_launch_chunks(monkeypatch, inverse, wide):
     """A multi-pass batch larger than one launch's grid holds (65535 transforms: the batch index
     is the grid's y dimension) runs in chunks; the transforms on both sides of the chunk boundary,
     with input and output distances that differ from the size and from each other, match FP64.
-    65537 transforms of 2^15 points: 16 GiB in and 16 GiB out."""
+    65537 transforms of 2^15 points: 16 GiB in and 16 GiB out. `wide`: the plan's split is forced
+    to 2^7 x 2^8 with 64 columns per tile and the halves layout, so the input pass runs 16384-point
+    tiles (1024-thread workgroups)."""
     import numpy as np
 
     L, batch = 15, 65537
     n = 1 << L
     in_dist, out_dist = n + 8, n + 24
-    print(f"chunked batch n=2^{L} x{batch} inverse={inverse}", flush=True)
+    print(f"chunked batch n=2^{L} x{batch} inverse={inverse} wide={wide}", flush=True)
+    if wide:
+        monkeypatch.setenv("ONEFFT_PASS_SPLIT", "7,8")
+        monkeypatch.setenv("ONEFFT_PASS_COLS", "64,64")
+        monkeypatch.setenv("ONEFFT_PASS_LAYOUT", "2")
+    plan = F.Plan(n)  # a plan of its own: the environment is read at creation
+    for k in ("ONEFFT_PASS_SPLIT", "ONEFFT_PASS_COLS", "ONEFFT_PASS_LAYOUT"):
+        monkeypatch.delenv(k, raising=False)
     dev = torch.device("cuda:0")
     g = torch.Generator(device=dev)
     g.manual_seed(7)
     x = torch.randn(batch * in_dist, dtype=torch.complex64, device=dev, generator=g)
     y = torch.zeros(batch * out_dist, dtype=torch.complex64, device=dev)
-    F.plan(n).batch_ptr(x.data_ptr(), y.data_ptr(), batch, in_dist, out_dist, inverse, stream())
+    plan.batch_ptr(x.data_ptr(), y.data_ptr(), batch, in_dist, out_dist, inverse, stream())
     torch.cuda.synchronize()
+    plan.close()
     for b in (0, 1, 65534, 65535, 65536):
         xs = x[b * in_dist: b * in_dist + n].cpu().numpy().astype(np.complex128)
         want = np.fft.ifft(xs) * n if inverse else np.fft.fft(xs)

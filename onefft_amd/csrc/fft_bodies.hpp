@@ -237,11 +237,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, const cf *__restric
             });
             if constexpr (!FIRST) {
                 // element e = j + T q + STRIDE r gets w_n^((e low) << tw_shift), formed in FP64
-                const uint32_t lo_mask = (1u << a.lo_bits) - 1;
-                auto W = [&](uint32_t x) {
-                    x <<= a.tw_shift;
-                    return mul64(a.tw_hi[x >> a.lo_bits], a.tw_lo[x & lo_mask]);
-                };
+                auto W = [&](uint32_t x) { return two_level(a.tw_hi, a.tw_lo, a.lo_bits, x << a.tw_shift); };
                 cd wq = W(j * low);
                 const cd step_q = W(T * low);
                 const cd step_r = W(STRIDE * low);

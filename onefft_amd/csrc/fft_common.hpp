@@ -14,27 +14,10 @@
 
 #include <type_traits>
 
-// Helpers are host-callable too: the plan builder includes this header for the stage schedule.
-#define ONEFFT_HD __host__ __device__ __forceinline__
+#include "plan_tables.hpp"
 
 namespace onefft {
 
-// One complex value, 8-byte aligned so a load or store is one dwordx2.
-struct __attribute__((aligned(8))) cf {
-    float r, i;
-};
-
-// One complex value in FP64, for twiddle products that are rounded to FP32 only once.
-struct __attribute__((aligned(16))) cd {
-    double r, i;
-};
-
-ONEFFT_HD cf mk(float r, float i) {
-    cf c;
-    c.r = r;
-    c.i = i;
-    return c;
-}
 ONEFFT_HD cf add(cf a, cf b) { return mk(a.r + b.r, a.i + b.i); }
 ONEFFT_HD cf sub(cf a, cf b) { return mk(a.r - b.r, a.i - b.i); }
 // a * w
@@ -53,14 +36,6 @@ ONEFFT_HD cf maybe_conj(cf a) {
     if constexpr (CONJ) return mk(a.r, -a.i);
     else return a;
 }
-
-ONEFFT_HD cd mul64(cd a, cd b) {
-    cd c;
-    c.r = a.r * b.r - a.i * b.i;
-    c.i = a.r * b.i + a.i * b.r;
-    return c;
-}
-ONEFFT_HD cf round32(cd a) { return mk((float)a.r, (float)a.i); }
 
 // ---- compile-time loops -------------------------------------------------------------------
 
@@ -166,75 +141,6 @@ ONEFFT_HD void dft(cf *v) {
         dft<A>(z);
         static_for<0, A>([&](auto m) { v[k + B * m] = z[m]; });
     });
-}
-
-// ---- Stockham stage schedule ------------------------------------------------------------------
-//
-// An N-point transform in which every thread holds P points runs in stages. Stage s has radix
-// R_s (a divisor of P) and Ns = R_0 ... R_{s-1}; each thread runs P / R_s butterflies. Butterfly
-// b reads x[b + r N / R_s], multiplies input r by w_{Ns R_s}^{r (b mod Ns)}, transforms, and
-// writes output m to (b / Ns) Ns R_s + (b mod Ns) + m Ns. After the last stage the result is in
-// natural order (Stockham autosort; Govindaraju et al., SC 2008). The smaller radix, if any,
-// comes first.
-
-__host__ __device__ constexpr int stage_count(int log2n, int log2p) {
-    return log2n <= log2p ? 1 : log2n / log2p + (log2n % log2p ? 1 : 0);
-}
-__host__ __device__ constexpr int stage_log2r(int log2n, int log2p, int s) {
-    return log2n <= log2p ? log2n : (log2n % log2p && s == 0) ? log2n % log2p : log2p;
-}
-// log2 of Ns for stage s
-__host__ __device__ constexpr int stage_log2ns(int log2n, int log2p, int s) {
-    int l = 0;
-    for (int i = 0; i < s; i++) l += stage_log2r(log2n, log2p, i);
-    return l;
-}
-// Offset (in complex entries) of stage s's twiddles in the per-size table, and the table size.
-// Stage s stores w_{Ns R}^{r k} at [(r - 1) Ns + k] for r = 1..R-1, k = 0..Ns-1.
-__host__ __device__ constexpr uint32_t stage_tw_offset(int log2n, int log2p, int s) {
-    uint32_t off = 0;
-    for (int i = 1; i < s; i++) {
-        uint32_t r = 1u << stage_log2r(log2n, log2p, i);
-        off += (r - 1) << stage_log2ns(log2n, log2p, i);
-    }
-    return off;
-}
-__host__ __device__ constexpr uint32_t stage_tw_size(int log2n, int log2p) {
-    return stage_tw_offset(log2n, log2p, stage_count(log2n, log2p));
-}
-
-// Stages after the first with radix >= 4 can take one FP64 base w_{Ns R}^k per butterfly instead
-// of R - 1 table twiddles and form w^(r k), r = 2..R-1, by an FP64 recurrence, each rounded once
-// to FP32: 16 bytes of table per butterfly instead of 8 (R - 1), at ~6 FP64 instructions per
-// twiddle. Plans hold the FP64 bases of every such stage (w_{Ns R}^k for k < Ns at
-// stage_tw64_offset(.., s)); a kernel uses them for the stages with Ns >= 2^tw64_min_log2ns(level),
-// the level being a tuning choice: 0 none, 1 Ns >= 64 (the stages whose FP32 tables outgrow the
-// L1 cache), 2 Ns >= 16, 3 every such stage.
-__host__ __device__ constexpr bool stage_has_tw64(int log2n, int log2p, int s) {
-    return s > 0 && stage_log2r(log2n, log2p, s) >= 2;
-}
-__host__ __device__ constexpr int tw64_min_log2ns(int level) {
-    return level <= 0 ? 64 : level == 1 ? 6 : level == 2 ? 4 : 0;
-}
-__host__ __device__ constexpr bool stage_uses_tw64(int log2n, int log2p, int s, int level) {
-    return stage_has_tw64(log2n, log2p, s) && stage_log2ns(log2n, log2p, s) >= tw64_min_log2ns(level);
-}
-__host__ __device__ constexpr uint32_t stage_tw64_offset(int log2n, int log2p, int s) {
-    uint32_t off = 0;
-    for (int i = 1; i < s; i++)
-        if (stage_has_tw64(log2n, log2p, i)) off += 1u << stage_log2ns(log2n, log2p, i);
-    return off;
-}
-__host__ __device__ constexpr uint32_t stage_tw64_size(int log2n, int log2p) {
-    return stage_tw64_offset(log2n, log2p, stage_count(log2n, log2p));
-}
-// the highest level that changes the kernel of this size (0: no stage can use FP64 bases)
-__host__ __device__ constexpr int tw64_levels(int log2n, int log2p) {
-    int top = 0;
-    for (int level = 1; level <= 3; level++)
-        for (int s = 1; s < stage_count(log2n, log2p); s++)
-            if (stage_uses_tw64(log2n, log2p, s, level) && !stage_uses_tw64(log2n, log2p, s, level - 1)) top = level;
-    return top;
 }
 
 // ---- LDS layout ---------------------------------------------------------------------------

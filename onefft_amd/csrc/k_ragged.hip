@@ -71,11 +71,7 @@ __device__ __forceinline__ void first_stage(const Column &io, uint32_t N, uint32
         });
     });
     if (io.twiddle) {
-        const uint32_t mask = (1u << io.lo_bits) - 1;
-        auto W = [&](uint32_t x) {
-            x <<= io.shift;
-            return mul64(io.hi[x >> io.lo_bits], io.lo[x & mask]);
-        };
+        auto W = [&](uint32_t x) { return two_level(io.hi, io.lo, io.lo_bits, x << io.shift); };
         const cd step = W(stride * io.low);
         static_for<0, BF>([&](auto q) {
             constexpr int qq = decltype(q)::value;

@@ -113,19 +113,11 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
             z.l2 = (uint32_t)l - z.l1;
             z.tw64_1 = stage64_table((int)z.l1);
             z.tw64_2 = stage64_table((int)z.l2);
-            z.lo_bits = ((uint32_t)l + 1) / 2;
+            z.lo_bits = two_level_lo_bits((unsigned)l);
             z.lo = (uint32_t)tw64.size();
-            for (uint64_t i = 0; i < (uint64_t(1) << z.lo_bits); i++) {
-                cd w;
-                root64(i, (unsigned)l, &w.r, &w.i);
-                tw64.push_back(w);
-            }
-            z.hi = (uint32_t)tw64.size();
-            for (uint64_t i = 0; i < (uint64_t(1) << (l - z.lo_bits)); i++) {
-                cd w;
-                root64(i << z.lo_bits, (unsigned)l, &w.r, &w.i);
-                tw64.push_back(w);
-            }
+            z.hi = z.lo + (1u << z.lo_bits);
+            tw64.resize(z.hi + (size_t(1) << (l - z.lo_bits)));
+            fill_two_level((unsigned)l, tw64.data() + z.lo, tw64.data() + z.hi);
         }
     }
 

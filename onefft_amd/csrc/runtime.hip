@@ -18,6 +18,7 @@
 #include "../../include/onefft.h"
 #include "fft_common.hpp"
 #include "kernels.hpp"
+#include "plan_tables.hpp"
 #include "runtime.hpp"
 
 using onefft::BatchVariant;
@@ -32,83 +33,9 @@ constexpr unsigned kMaxLog2 = 30;  // the reference's limit (src/onefft.c:1117-1
 constexpr unsigned kPlanStagingMaxLog2 = 22;
 }  // namespace
 
-namespace onefft {
-
-// ---- twiddles -------------------------------------------------------------------------------
-//
-// exp(-2 pi i k / 2^log2size) in FP64. The phase k / 2^log2size of a turn is held exactly as a
-// 0.64 fixed-point fraction; the angle is folded into the first octant before sin/cos and the
-// quadrant is applied exactly, so values are exactly symmetric and exact at multiples of an
-// eighth turn. Same steps as the reference's onefft_root_double (src/onefft.c:112-135), which
-// fixes the twiddle values bit for bit once rounded to FP32.
-void root64(uint64_t k, unsigned log2size, double *re, double *im) {
-    const uint64_t phase = log2size ? k << (64 - log2size) : 0;
-    const unsigned quadrant = (unsigned)(phase >> 62);
-    const uint64_t in_quadrant = phase & ((uint64_t(1) << 62) - 1);
-    const double unit = 6.283185307179586476925286766559 / 18446744073709551616.0;  // radians per 2^-64 turn
-    double c, s;
-    if (in_quadrant > (uint64_t(1) << 61)) {
-        const double a = (double)((uint64_t(1) << 62) - in_quadrant) * unit;
-        c = std::sin(a);
-        s = std::cos(a);
-    } else {
-        const double a = (double)in_quadrant * unit;
-        c = std::cos(a);
-        s = std::sin(a);
-    }
-    switch (quadrant) {  // (c - i s) (-i)^quadrant
-    case 0: *re = c, *im = -s; break;
-    case 1: *re = -s, *im = -c; break;
-    case 2: *re = -c, *im = s; break;
-    default: *re = s, *im = c; break;
-    }
-}
-
-cf root32(uint64_t k, unsigned log2size) {
-    double re, im;
-    root64(k, log2size, &re, &im);
-    cf c;
-    c.r = (float)re;
-    c.i = (float)im;
-    return c;
-}
-
-// Stage twiddles of an in-LDS transform of 2^log2n points, P = 2^log2p points per thread, in
-// the order the kernels read them (fft_common.hpp: stage_tw_offset).
-void fill_stage_table(int log2n, int log2p, cf *table) {
-    const int stages = onefft::stage_count(log2n, log2p);
-    for (int s = 1; s < stages; s++) {
-        const uint32_t off = onefft::stage_tw_offset(log2n, log2p, s);
-        const int lr = onefft::stage_log2r(log2n, log2p, s);
-        const int lns = onefft::stage_log2ns(log2n, log2p, s);
-        const uint32_t r_count = 1u << lr, ns = 1u << lns;
-        for (uint32_t r = 1; r < r_count; r++)
-            for (uint32_t k = 0; k < ns; k++) table[off + (r - 1) * ns + k] = root32((uint64_t)r * k, lns + lr);
-    }
-}
-
-// FP64 bases w_{Ns R}^k, k < Ns, of every stage that can use them (fft_common.hpp: stage_has_tw64).
-void fill_stage_table64(int log2n, int log2p, cd *table) {
-    const int stages = stage_count(log2n, log2p);
-    for (int s = 1; s < stages; s++) {
-        if (!stage_has_tw64(log2n, log2p, s)) continue;
-        const uint32_t off = stage_tw64_offset(log2n, log2p, s);
-        const int lr = stage_log2r(log2n, log2p, s);
-        const int lns = stage_log2ns(log2n, log2p, s);
-        for (uint32_t k = 0; k < (1u << lns); k++) root64(k, lns + lr, &table[off + k].r, &table[off + k].i);
-    }
-
-}  // namespace onefft
-
 namespace {
 
-using onefft::fill_stage_table64;
-using onefft::root64;
-using onefft::fill_stage_table;
-
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
-
-}
 
 // Pass split of a multi-pass size: log2 N_1 .. log2 N_p (N_1 is transformed last), the columns
 // per workgroup of each factor's pass, and the passes' LDS layout. The environment variable ONEFFT_PASS_SPLIT
@@ -340,16 +267,11 @@ extern "C" onefft_plan *onefft_plan_create(size_t n) {
     } else {
         plan->multi = true;
         // two-level FP64 table of w_n, shared by both configurations
-        const unsigned lo_bits = (log2n + 1) / 2;
+        const unsigned lo_bits = two_level_lo_bits(log2n);
         const size_t lo_count = (size_t)1 << lo_bits, hi_count = (size_t)1 << (log2n - lo_bits);
         off_lo = reserve(lo_count * sizeof(cd));
         off_hi = reserve(hi_count * sizeof(cd));
-        {
-            cd *lo = reinterpret_cast<cd *>(image.data() + off_lo);
-            for (size_t i = 0; i < lo_count; i++) root64(i, log2n, &lo[i].r, &lo[i].i);
-            cd *hi = reinterpret_cast<cd *>(image.data() + off_hi);
-            for (size_t i = 0; i < hi_count; i++) root64((uint64_t)i << lo_bits, log2n, &hi[i].r, &hi[i].i);
-        }
+        fill_two_level(log2n, reinterpret_cast<cd *>(image.data() + off_lo), reinterpret_cast<cd *>(image.data() + off_hi));
         for (int r = 0; r < 2; r++) {
             int factors[kMaxPasses], cols[kMaxPasses], layout = kPadded;
             const int p = choose_split(log2n, r, factors, cols, &layout);

@@ -31,14 +31,6 @@ bool staging_reserve(int device, size_t n);
 // The synchronous drop-in call: host or device pointers of any alignment.
 int execute_host_call(const onefft_plan *plan, const float *in, float *out, bool inverse);
 
-// exp(-2 pi i k / 2^log2size) in FP64 (octant-folded), and its FP32 rounding.
-void root64(uint64_t k, unsigned log2size, double *re, double *im);
-cf root32(uint64_t k, unsigned log2size);
-// Stage twiddles of an in-LDS transform of 2^log2n points with 2^log2p points per thread, and
-// the FP64 bases of its wide stages (fft_common.hpp: stage_uses_tw64).
-void fill_stage_table(int log2n, int log2p, cf *table);
-void fill_stage_table64(int log2n, int log2p, cd *table);
-
 // The cache's plan for size n (created on first use), or nullptr.
 const onefft_plan *cached_plan(size_t n);
 

@@ -16,8 +16,16 @@
  * On this implementation the transform runs on the GPU. The drop-in functions accept host
  * pointers (copied to and from the device around the transform, which then dominates the cost)
  * and device pointers (hipMalloc'd memory, transformed in place on the device); they return when
- * `out` holds the result. The second part of the header adds asynchronous device-pointer,
- * batched and ragged entry points on a caller-supplied HIP stream.
+ * `out` holds the result. A call with a device-memory argument is ordered like a synchronous
+ * hipMemcpy: after all work queued before it on the null (default) stream and on blocking
+ * streams; work on other non-blocking streams must be synchronised by the caller first. The
+ * second part of the header adds asynchronous device-pointer, batched and ragged entry points on
+ * a caller-supplied HIP stream.
+ *
+ * Host-pointer and misaligned device-pointer calls stage data through a process-wide pool of
+ * ONEFFT_STAGING_SLOTS (default 4) slots per GPU, which onefft_plan_create and
+ * onefft_cache_prepare size: after them, calls of their size allocate nothing and take no lock on
+ * any thread, as long as at most that many calls run at once (reference include/onefft.h:66-71).
  */
 #ifndef ONEFFT_H
 #define ONEFFT_H
@@ -43,23 +51,25 @@ extern "C" {
 int onefft_forward(const float *in, float *out, size_t n);
 int onefft_inverse(const float *in, float *out, size_t n);
 
-/* Creates the plan for size n now (reference include/onefft.h:79-82), and sizes the calling
- * thread's device staging buffers for n, so later calls of that size on this thread allocate
- * nothing. Returns 0, or -1 if n is invalid or on allocation failure. */
+/* Creates the plan for size n now (reference include/onefft.h:79-82) and sizes the staging pool
+ * for host-pointer calls of n points, so that later onefft_forward/inverse calls of that size
+ * allocate nothing and take no lock on any thread (at most ONEFFT_STAGING_SLOTS at once). Returns
+ * 0, or -1 if n is invalid or on allocation failure. */
 int onefft_cache_prepare(size_t n);
 
-/* Frees every plan created by the functions above (reference include/onefft.h:84-86). Must not
- * run while another thread is inside one of them. */
+/* Frees every plan created by the functions above and the staging pool (reference
+ * include/onefft.h:84-86). Must not run while another thread is inside any function of this
+ * library. */
 void onefft_cache_clear(void);
 
 /* A plan holds the device tables for one size and serves both directions
  * (reference include/onefft.h:88-109). */
 typedef struct onefft_plan onefft_plan;
 
-/* NULL if n is not a power of two from 1 to 2^30 or if the device allocation fails. The plan
- * belongs to the HIP device that is current when it is created. For n <= 2^22 it also sets up
- * the calling thread's stream and staging buffers for host-pointer calls of this size (larger
- * plans leave that to the first such call on each thread, or to onefft_cache_prepare). */
+/* NULL if n is not a power of two from 1 to 2^30 or if an allocation fails. The plan belongs to
+ * the HIP device that is current when it is created. It also creates that device's staging pool
+ * and, for n <= 2^22, sizes it for host-pointer calls of this size (larger plans leave the
+ * buffers to the first such call, or to onefft_cache_prepare). */
 onefft_plan *onefft_plan_create(size_t n);
 
 /* Frees a plan; NULL is allowed. */
@@ -67,8 +77,9 @@ void onefft_plan_destroy(onefft_plan *plan);
 
 /* Transform from `in` to `out` (host or device pointers, no alignment required, no overlap).
  * The plan is never modified: any number of threads may use one plan at once, in either
- * direction. The first call on a thread creates that thread's stream and staging buffers. A HIP
- * failure leaves a message for onefft_last_error(). */
+ * direction. Neither allocates nor locks once the staging pool serves the plan's size (see
+ * onefft_plan_create) and at most ONEFFT_STAGING_SLOTS calls run at once. A failure leaves `out`
+ * untouched and a message for onefft_last_error(). */
 void onefft_plan_forward(const onefft_plan *plan, const float *in, float *out);
 void onefft_plan_inverse(const onefft_plan *plan, const float *in, float *out);
 
@@ -135,7 +146,8 @@ size_t onefft_ragged_task_count(const onefft_ragged_plan *plan);
  * time: executions of one plan must be ordered (same stream or synchronised). Returns 0 or -1. */
 int onefft_ragged_execute(const onefft_ragged_plan *plan, const float *d_in, float *d_out, struct ihipStream_t *stream);
 
-/* The same result with one device plan call per item, for comparison. */
+/* The same result with one device plan call per item, for comparison; creates the plan cache's
+ * plan of each size on first use. */
 int onefft_ragged_execute_per_item(const onefft_ragged_plan *plan, const float *d_in, float *d_out,
                                    struct ihipStream_t *stream);
 

@@ -18,8 +18,8 @@ namespace onefft {
 // half the rate the DMA engine moves pinned memory over the host link, so a single-threaded ring
 // is bound by its memcpy. A process-wide fork-join pool of ONEFFT_COPY_THREADS helper threads
 // (default 4, 0 for none) copies 1 MiB pieces of a chunk together with the calling thread; one
-// job runs at a time (calls from several threads take turns). Each job lives on its caller's stack
-// and is released only once no helper refers to it.
+// job runs at a time, and a call from another thread meanwhile copies alone instead of waiting.
+// Each job lives on its caller's stack and is released only once no helper refers to it.
 class CopyPool {
 public:
     static CopyPool &get() {
@@ -39,7 +39,13 @@ public:
             std::memcpy(dst, src, bytes);
             return;
         }
-        std::lock_guard<std::mutex> turn(turn_);
+        // the helpers serve one job at a time; a call that finds them taken copies alone rather
+        // than wait for another caller's copy
+        std::unique_lock<std::mutex> turn(turn_, std::try_to_lock);
+        if (!turn.owns_lock()) {
+            std::memcpy(dst, src, bytes);
+            return;
+        }
         Job job{static_cast<char *>(dst), static_cast<const char *>(src), bytes, (bytes + kPiece - 1) / kPiece};
         {
             std::lock_guard<std::mutex> lk(m_);

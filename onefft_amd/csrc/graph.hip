@@ -32,7 +32,7 @@ onefft_graph *capture(int device, Record record) {
     if (!g) return nullptr;
     g->device = device;
     hipStream_t s = nullptr;
-    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    hipError_t e = onefft::stream_create(&s);
     if (e != hipSuccess) {
         onefft::set_error("graph stream", e);
         delete g;

@@ -200,8 +200,7 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
     std::memcpy(image.data() + o_tasks, tasks.data(), tasks.size() * sizeof(RaggedTask));
     std::memcpy(image.data() + o_off, item_off.data(), item_off.size() * sizeof(uint64_t));
     std::memcpy(image.data() + o_tw64, tw64.data(), tw64.size() * sizeof(cd));
-    if (hipMalloc(&plan->dmem, bytes) != hipSuccess) {
-        (void)hipGetLastError();
+    if (dev_malloc(&plan->dmem, bytes) != hipSuccess) {
         set_error("ragged plan allocation failed");
         delete plan;
         return nullptr;
@@ -226,13 +225,6 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
     a.spin_limit = 1u << 24;
     if (const char *env = std::getenv("ONEFFT_RAGGED_SPIN_LIMIT")) a.spin_limit = (uint32_t)std::strtoul(env, nullptr, 10);
     plan->grid = ragged_resident_grid();
-    // the per-item comparison uses the cached plan of every size
-    for (int l = 0; l <= kRaggedMaxLog2; l++)
-        if (used[l] && !cached_plan((size_t)1 << l)) {
-            (void)hipFree(plan->dmem);
-            delete plan;
-            return nullptr;
-        }
     return plan;
 }
 

@@ -27,9 +27,10 @@ using onefft::cf;
 
 namespace {
 constexpr unsigned kMaxLog2 = 30;  // the reference's limit (src/onefft.c:1117-1121)
-// Plans up to this size also set up the creating thread's host-call staging (2 x 32 MiB of device
-// memory at most), so that thread's host-pointer calls allocate nothing; larger plans, often
-// used with device pointers only, leave it to the first host-pointer call or onefft_cache_prepare.
+// Plans up to this size also size the staging pool's buffers for host-pointer calls of their size
+// (per slot at most 2 x 32 MiB of device memory and a 16 MiB pinned ring), so those calls allocate
+// nothing on any thread; larger plans, often used with device pointers only, create the pool's
+// slots (streams) but leave the buffers to the first host-pointer call or onefft_cache_prepare.
 constexpr unsigned kPlanStagingMaxLog2 = 22;
 }  // namespace
 
@@ -309,8 +310,7 @@ extern "C" onefft_plan *onefft_plan_create(size_t n) {
         }
     }
 
-    if (hipMalloc(&plan->dmem, image.size()) != hipSuccess) {
-        (void)hipGetLastError();
+    if (dev_malloc(&plan->dmem, image.size()) != hipSuccess) {
         set_error("plan table allocation failed");
         delete plan;
         return nullptr;
@@ -336,7 +336,11 @@ extern "C" onefft_plan *onefft_plan_create(size_t n) {
             }
         }
     }
-    if (log2n <= kPlanStagingMaxLog2) (void)staging_reserve(plan->device, n);
+    if (!staging_prepare(plan->device, n, log2n <= kPlanStagingMaxLog2)) {
+        (void)hipFree(plan->dmem);
+        delete plan;
+        return nullptr;
+    }
     return plan;
 }
 
@@ -469,9 +473,15 @@ extern "C" int onefft_inverse(const float *in, float *out, size_t n) {
 extern "C" int onefft_cache_prepare(size_t n) {
     const onefft_plan *plan = cached_plan(n);
     if (!plan) return -1;
-    return staging_reserve(plan->device, n) ? 0 : -1;
+    DeviceGuard g(plan->device);
+    if (!g.ok) {
+        set_error("cannot select the plan's device");
+        return -1;
+    }
+    return staging_prepare(plan->device, n, true) ? 0 : -1;
 }
 
 extern "C" void onefft_cache_clear(void) {
     for (auto &slot : g_cache) onefft_plan_destroy(slot.exchange(nullptr, std::memory_order_acq_rel));
+    staging_release_all();
 }

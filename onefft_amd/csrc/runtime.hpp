@@ -24,9 +24,19 @@ void set_error(const char *what);
 int execute_device(const onefft_plan *plan, const cf *d_in, cf *d_out, uint64_t batch, uint64_t in_dist,
                    uint64_t out_dist, bool inverse, hipStream_t stream);
 
-// Make sure the calling thread has a stream and device staging for transforms of n points on
-// `device`. false on failure.
-bool staging_reserve(int device, size_t n);
+// Sizes the staging pool of `device` (staging.hip): creates its slots (ONEFFT_STAGING_SLOTS, each
+// with a stream and events) and, with `buffers`, gives every slot what a host-pointer call of n
+// points needs, so that such calls allocate nothing. false on failure.
+bool staging_prepare(int device, size_t n, bool buffers);
+// Frees every staging slot of every device (onefft_cache_clear).
+void staging_release_all();
+
+// The library's allocations (alloc.hip): hipMalloc, hipHostMalloc, a non-blocking stream and a
+// timing-free event, each of which the ONEFFT_FAIL_ALLOC switch can make fail.
+hipError_t dev_malloc(void **p, size_t bytes);
+hipError_t host_malloc(void **p, size_t bytes, unsigned flags);
+hipError_t stream_create(hipStream_t *s);
+hipError_t event_create(hipEvent_t *e);
 
 // The synchronous drop-in call: host or device pointers of any alignment.
 int execute_host_call(const onefft_plan *plan, const float *in, float *out, bool inverse);
@@ -48,9 +58,6 @@ struct TuneEntry {
     int pass_layout = 0;         // kMulti: LDS layout of every pass (kPadded or kHalves)
 };
 const TuneEntry &tune_lookup(unsigned log2n, int regime);
-
-// The calling thread's stream on `device` (created on first use), or nullptr on failure.
-hipStream_t thread_stream(int device);
 
 // Makes `device` current for the guard's lifetime and restores the previous device after; `ok`
 // is false if the device could not be selected.

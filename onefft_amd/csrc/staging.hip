@@ -2,14 +2,23 @@
 // or device pointers of any alignment (the reference accepts any alignment,
 // include/onefft.h:61-64; its tests misalign by 4 and 12 bytes, tests/test_onefft.c:118-119).
 //
-// Each calling thread owns a stream, a pair of device staging buffers (grown to the largest size
-// it has transformed) and a ring of pinned host chunks, all kept until the thread exits. Plans
-// stay read-only, so threads never contend (reference include/onefft.h:102-104). Per argument:
+// Staging lives in a process-wide pool per device of slots, each one caller's worth: a stream, a
+// pair of device staging buffers, a ring of pinned host chunks and a coherent pinned buffer. A
+// call claims a free slot with one atomic exchange (trying first the slot its thread used last)
+// and returns it when done, so calls never wait for each other and plans stay read-only
+// (reference include/onefft.h:102-104). onefft_plan_create and onefft_cache_prepare size the
+// pool (ONEFFT_STAGING_SLOTS slots, default 4) for their size, after which calls of that size
+// allocate nothing on any thread while at most that many run at once (the reference's promise,
+// README.md:35-41, include/onefft.h:66-71). Only a call that finds every slot busy, or a slot
+// smaller than it needs, allocates (a new slot, or a larger buffer). Slots are released by
+// onefft_cache_clear, never at thread or process exit.
+//
+// Per argument:
 //   * device memory, 8-byte aligned: transformed where it lies;
 //   * device memory, misaligned: copied device to device into the staging buffer and back;
 //   * pinned host memory: copied by DMA straight to and from the staging buffers;
 //   * ordinary (pageable) host memory, transforms of one LDS pass (n <= 2^14, 128 KiB): copied by
-//     the CPU into a per-thread coherent pinned buffer that the kernel reads and writes directly
+//     the CPU into the slot's coherent pinned buffer, which the kernel reads and writes directly
 //     over the host link (no DMA transfers, whose set-up latency dominates at these sizes);
 //   * ordinary (pageable) host memory up to 4 MiB: HIP's own pageable copies (lower set-up latency
 //     than the ring below at these sizes);
@@ -19,10 +28,18 @@
 //     and the link run at the same time; the CPU copies are split over a small thread pool
 //     (CopyPool). (HIP's own pageable copies run at ~30 GB/s host to device and ~18 GB/s back on
 //     MI355X hosts; pinned DMA at ~57 GB/s each way; one copying thread ~28 GB/s.)
+// A call with a device-memory argument first orders its stream after all work queued so far on
+// the null stream (and, through it, on every blocking stream), like a synchronous hipMemcpy, so a
+// producer kernel on the caller's default stream finishes before the transform reads.
 // The output is written only after the transform has completed without error.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <cstdlib>
 #include <cstring>
+#include <mutex>
+#include <new>
+#include <thread>
 
 #include "../../include/onefft.h"
 #include "copy_pool.hpp"
@@ -32,42 +49,208 @@
 namespace onefft {
 namespace {
 
-constexpr size_t kChunk = size_t(4) << 20;  // bytes per pinned slot
-constexpr int kSlots = 4;                   // slots in the ring: 16 MiB pinned per thread
+constexpr size_t kChunk = size_t(4) << 20;  // bytes per pinned ring chunk
+constexpr int kRing = 4;                    // chunks in a ring: 16 MiB pinned per slot
 constexpr size_t kDirectBytes = size_t(8) << kMaxSinglePassLog2;  // one LDS-pass transform
 constexpr size_t kRingMinBytes = kChunk;                          // smaller: HIP's pageable copies
+constexpr int kMaxSlots = 64;
+constexpr int kMaxDevices = 16;
+constexpr int kDefaultSlots = 4;
 
+// what a call needs besides the stream
+enum Need : unsigned { kNeedBuf = 1, kNeedRing = 2, kNeedDirect = 4 };
 
-struct Staging {
-    int device = -1;
+unsigned needs_for_size(size_t bytes) {
+    return kNeedBuf | (bytes >= kRingMinBytes ? kNeedRing : 0u) | (bytes <= kDirectBytes ? kNeedDirect : 0u);
+}
+
+// One caller's staging, held by one call at a time (busy).
+struct Slot {
+    std::atomic<bool> busy{false};
     hipStream_t stream = nullptr;
+    hipEvent_t order = nullptr;  // recorded on the null stream before a device-memory call
+    hipEvent_t ev[kRing] = {};
     void *buf = nullptr;  // 2 * cap bytes: input then output
     size_t cap = 0;
-    char *pinned = nullptr;  // kSlots * kChunk bytes of pinned host memory
-    hipEvent_t ev[kSlots] = {};
+    char *pinned = nullptr;      // kRing * kChunk bytes of pinned host memory
     char *direct = nullptr;      // 2 * kDirectBytes of coherent pinned host memory: input, output
     char *direct_dev = nullptr;  // the same memory as the device addresses it
+
+    // the stream and events; all or nothing
+    bool create() {
+        if (stream_create(&stream) != hipSuccess) {
+            set_error("staging stream creation failed");
+            return false;
+        }
+        bool ok = event_create(&order) == hipSuccess;
+        for (int i = 0; ok && i < kRing; i++) ok = event_create(&ev[i]) == hipSuccess;
+        if (!ok) {
+            set_error("staging event creation failed");
+            release();
+        }
+        return ok;
+    }
+
+    bool serves(size_t bytes, unsigned need) const {
+        return (!(need & kNeedBuf) || cap >= bytes) && (!(need & kNeedRing) || pinned) &&
+               (!(need & kNeedDirect) || direct);
+    }
+
+    // grows the slot to serve `need` for transforms of `bytes` (allocates); a failure leaves the
+    // slot as it was
+    bool ensure(size_t bytes, unsigned need) {
+        if ((need & kNeedRing) && !pinned) {
+            void *p = nullptr;
+            if (host_malloc(&p, kRing * kChunk, hipHostMallocDefault) != hipSuccess) {
+                set_error("pinned staging allocation failed");
+                return false;
+            }
+            pinned = static_cast<char *>(p);
+        }
+        if ((need & kNeedDirect) && !direct) {
+            void *p = nullptr, *d = nullptr;
+            if (host_malloc(&p, 2 * kDirectBytes, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+                set_error("pinned staging allocation failed");
+                return false;
+            }
+            if (hipHostGetDevicePointer(&d, p, 0) != hipSuccess) {
+                (void)hipGetLastError();
+                (void)hipHostFree(p);
+                set_error("pinned staging mapping failed");
+                return false;
+            }
+            direct = static_cast<char *>(p);
+            direct_dev = static_cast<char *>(d);
+        }
+        if ((need & kNeedBuf) && cap < bytes) {
+            void *p = nullptr;
+            if (dev_malloc(&p, 2 * bytes) != hipSuccess) {
+                set_error("staging allocation failed");
+                return false;
+            }
+            if (buf) {
+                (void)hipStreamSynchronize(stream);
+                (void)hipFree(buf);
+            }
+            buf = p;
+            cap = bytes;
+        }
+        return true;
+    }
 
     void release() {
         if (stream) (void)hipStreamSynchronize(stream);
         if (buf) (void)hipFree(buf);
         if (pinned) (void)hipHostFree(pinned);
         if (direct) (void)hipHostFree(direct);
-        direct = direct_dev = nullptr;
+        if (order) (void)hipEventDestroy(order);
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
-        for (auto &e : ev) e = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
-        buf = nullptr;
-        pinned = nullptr;
         stream = nullptr;
+        order = nullptr;
+        for (auto &e : ev) e = nullptr;
+        buf = nullptr;
         cap = 0;
-        device = -1;
+        pinned = direct = direct_dev = nullptr;
     }
-    ~Staging() { release(); }
 };
 
-thread_local Staging t_stage;
+// The slots of one device. Slots [0, count) exist; they are created under `grow` and published by
+// the release store of count, then only claimed and returned.
+struct Pool {
+    std::atomic<int> count{0};
+    Slot slots[kMaxSlots];
+    std::mutex grow;  // slot creation, onefft_cache_prepare and onefft_cache_clear
+};
+
+// Never destroyed: the HIP runtime may already be gone when static destructors run.
+Pool *g_pools[kMaxDevices];
+std::mutex g_pools_mutex;
+
+Pool *pool(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    // pools are created once and never freed; the mutex covers the creation only
+    std::lock_guard<std::mutex> lk(g_pools_mutex);
+    Pool *p = __atomic_load_n(&g_pools[device], __ATOMIC_RELAXED);
+    if (!p) {
+        p = new (std::nothrow) Pool;
+        __atomic_store_n(&g_pools[device], p, __ATOMIC_RELEASE);
+    }
+    return p;
+}
+
+Pool *pool_fast(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    Pool *p = __atomic_load_n(&g_pools[device], __ATOMIC_ACQUIRE);
+    return p ? p : pool(device);
+}
+
+int slots_wanted() {
+    const char *env = std::getenv("ONEFFT_STAGING_SLOTS");
+    const int v = env ? std::atoi(env) : kDefaultSlots;
+    return v < 1 ? 1 : v > kMaxSlots ? kMaxSlots : v;
+}
+
+thread_local int t_hint = 0;
+
+// A new slot, claimed by the caller; under p.grow.
+Slot *add_slot(Pool &p, bool claimed) {
+    const int m = p.count.load(std::memory_order_relaxed);
+    if (m >= kMaxSlots) {
+        set_error("every staging slot is busy");
+        return nullptr;
+    }
+    Slot &s = p.slots[m];
+    if (!s.create()) return nullptr;
+    s.busy.store(claimed, std::memory_order_relaxed);
+    p.count.store(m + 1, std::memory_order_release);
+    return &s;
+}
+
+// A free slot of `device`, claimed by the caller (nullptr on failure). Lock-free unless every
+// slot is busy.
+Slot *claim(int device) {
+    Pool *p = pool_fast(device);
+    if (!p) {
+        set_error("no staging pool for this device");
+        return nullptr;
+    }
+    for (int attempt = 0;; attempt++) {
+        const int n = p->count.load(std::memory_order_acquire);
+        for (int i = 0; i < n; i++) {
+            const int idx = (t_hint + i) % n;
+            Slot &s = p->slots[idx];
+            if (!s.busy.load(std::memory_order_relaxed) && !s.busy.exchange(true, std::memory_order_acquire)) {
+                t_hint = idx;
+                return &s;
+            }
+        }
+        if (n < kMaxSlots) {
+            std::lock_guard<std::mutex> lk(p->grow);
+            if (p->count.load(std::memory_order_relaxed) != n) continue;  // another thread added one
+            Slot *s = add_slot(*p, true);
+            if (s) t_hint = n;
+            return s;
+        }
+        // kMaxSlots calls at once: wait for one to finish
+        if (attempt > 1000000) {
+            set_error("every staging slot is busy");
+            return nullptr;
+        }
+        std::this_thread::yield();
+    }
+}
+
+struct Claim {
+    Slot *s;
+    explicit Claim(Slot *slot) : s(slot) {}
+    ~Claim() {
+        if (s) s->busy.store(false, std::memory_order_release);
+    }
+    Claim(const Claim &) = delete;
+    Claim &operator=(const Claim &) = delete;
+};
 
 enum class Mem { kPageable, kPinned, kDevice };
 
@@ -82,56 +265,13 @@ Mem memory_kind(const void *p) {
     return Mem::kPageable;
 }
 
-// the pinned ring and its events; all or nothing (s.pinned is set last)
-bool ring_reserve(Staging &s) {
-    if (s.pinned) return true;
-    for (auto &e : s.ev) {
-        if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
-            (void)hipGetLastError();
-            e = nullptr;
-            set_error("staging event creation failed");
-            return false;
-        }
-    }
-    char *p = nullptr;
-    if (hipHostMalloc(reinterpret_cast<void **>(&p), kSlots * kChunk, hipHostMallocDefault) != hipSuccess) {
-        (void)hipGetLastError();
-        set_error("pinned staging allocation failed");
-        return false;
-    }
-    s.pinned = p;
-    return true;
-}
-
-// the coherent pinned buffer pair the single-pass kernels read and write in place
-bool direct_reserve(Staging &s) {
-    if (s.direct) return true;
-    char *p = nullptr;
-    void *d = nullptr;
-    if (hipHostMalloc(reinterpret_cast<void **>(&p), 2 * kDirectBytes,
-                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
-        (void)hipGetLastError();
-        set_error("pinned staging allocation failed");
-        return false;
-    }
-    if (hipHostGetDevicePointer(&d, p, 0) != hipSuccess) {
-        (void)hipGetLastError();
-        (void)hipHostFree(p);
-        set_error("pinned staging mapping failed");
-        return false;
-    }
-    s.direct = p;
-    s.direct_dev = static_cast<char *>(d);
-    return true;
-}
-
 // pageable host -> device, through the pinned ring; asynchronous on s.stream when it returns
-bool stage_in(Staging &s, void *d_dst, const void *src, size_t bytes) {
+bool stage_in(Slot &s, void *d_dst, const void *src, size_t bytes) {
     const size_t chunks = (bytes + kChunk - 1) / kChunk;
     for (size_t i = 0; i < chunks; i++) {
-        const int slot = int(i % kSlots);
+        const int slot = int(i % kRing);
         const size_t off = i * kChunk, len = bytes - off < kChunk ? bytes - off : kChunk;
-        if (i >= size_t(kSlots)) {
+        if (i >= size_t(kRing)) {
             const hipError_t e = hipEventSynchronize(s.ev[slot]);
             if (e != hipSuccess) {
                 set_error("input copy", e);
@@ -152,17 +292,17 @@ bool stage_in(Staging &s, void *d_dst, const void *src, size_t bytes) {
 
 // device -> pageable host, through the pinned ring, after everything queued on s.stream;
 // synchronous. Nothing is written to dst unless the queued work completed without error.
-bool stage_out(Staging &s, void *dst, const void *d_src, size_t bytes) {
+bool stage_out(Slot &s, void *dst, const void *d_src, size_t bytes) {
     const size_t chunks = (bytes + kChunk - 1) / kChunk;
     auto enqueue = [&](size_t i) {
-        const int slot = int(i % kSlots);
+        const int slot = int(i % kRing);
         const size_t off = i * kChunk, len = bytes - off < kChunk ? bytes - off : kChunk;
         hipError_t e = hipMemcpyAsync(s.pinned + slot * kChunk, static_cast<const char *>(d_src) + off, len,
                                       hipMemcpyDeviceToHost, s.stream);
         if (e == hipSuccess) e = hipEventRecord(s.ev[slot], s.stream);
         return e;
     };
-    for (size_t i = 0; i < chunks && i < size_t(kSlots); i++) {
+    for (size_t i = 0; i < chunks && i < size_t(kRing); i++) {
         const hipError_t e = enqueue(i);
         if (e != hipSuccess) {
             set_error("output copy", e);
@@ -171,7 +311,7 @@ bool stage_out(Staging &s, void *dst, const void *d_src, size_t bytes) {
         }
     }
     for (size_t i = 0; i < chunks; i++) {
-        const int slot = int(i % kSlots);
+        const int slot = int(i % kRing);
         const size_t off = i * kChunk, len = bytes - off < kChunk ? bytes - off : kChunk;
         const hipError_t e = hipEventSynchronize(s.ev[slot]);
         if (e != hipSuccess) {
@@ -180,8 +320,8 @@ bool stage_out(Staging &s, void *dst, const void *d_src, size_t bytes) {
             return false;
         }
         CopyPool::get().copy(static_cast<char *>(dst) + off, s.pinned + slot * kChunk, len);
-        if (i + kSlots < chunks) {
-            const hipError_t e2 = enqueue(i + kSlots);
+        if (i + kRing < chunks) {
+            const hipError_t e2 = enqueue(i + kRing);
             if (e2 != hipSuccess) {
                 set_error("output copy", e2);
                 (void)hipStreamSynchronize(s.stream);
@@ -192,61 +332,31 @@ bool stage_out(Staging &s, void *dst, const void *d_src, size_t bytes) {
     return true;
 }
 
-}  // namespace
-
-hipStream_t thread_stream(int device) {
-    Staging &s = t_stage;
-    if (s.device != device) {
-        s.release();
-        s.device = device;
-    }
-    if (!s.stream && hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess) {
-        (void)hipGetLastError();
-        s.stream = nullptr;
-        set_error("stream creation failed");
-    }
-    return s.stream;
-}
-
-bool staging_reserve(int device, size_t n) {
-    if (!thread_stream(device)) return false;
-    Staging &s = t_stage;
-    if (n && !ring_reserve(s)) return false;
-    if (n && n * 2 * sizeof(float) <= kDirectBytes && !direct_reserve(s)) return false;
-    const size_t bytes = n * 2 * sizeof(float);
-    if (s.cap >= bytes) return true;
-    if (s.buf) {
-        (void)hipStreamSynchronize(s.stream);
-        (void)hipFree(s.buf);
-        s.buf = nullptr;
-        s.cap = 0;
-    }
-    if (hipMalloc(&s.buf, 2 * bytes) != hipSuccess) {
-        (void)hipGetLastError();
-        s.buf = nullptr;
-        set_error("staging allocation failed");
-        return false;
-    }
-    s.cap = bytes;
-    return true;
-}
-
-int execute_host_call(const onefft_plan *plan, const float *in, float *out, bool inverse) {
+int transform(Slot &s, const onefft_plan *plan, const float *in, float *out, bool inverse) {
     const size_t n = onefft_plan_size(plan);
     const size_t bytes = n * 2 * sizeof(float);
-    int device = 0;
-    if (hipGetDevice(&device) != hipSuccess) {
-        set_error("no HIP device");
-        return -1;
-    }
     const Mem in_kind = memory_kind(in), out_kind = memory_kind(out);
     // device buffers are used where they lie when aligned; an input that overlaps the output
     // (which the API does not allow) is still copied first, so the result stays correct
     const bool in_direct = in_kind == Mem::kDevice && ((uintptr_t)in & 7) == 0 && disjoint(in, bytes, out, bytes);
     const bool out_direct = out_kind == Mem::kDevice && ((uintptr_t)out & 7) == 0;
-    if (!staging_reserve(device, (in_direct && out_direct) ? 0 : n)) return -1;
-    Staging &s = t_stage;
-    if (in_kind == Mem::kPageable && out_kind == Mem::kPageable && bytes <= kDirectBytes) {
+    const bool small_pageable = in_kind == Mem::kPageable && out_kind == Mem::kPageable && bytes <= kDirectBytes;
+    unsigned need = 0;
+    if (small_pageable) need = kNeedDirect;
+    else if (!(in_direct && out_direct))
+        need = kNeedBuf | ((in_kind == Mem::kPageable || out_kind == Mem::kPageable) && bytes >= kRingMinBytes ? kNeedRing : 0u);
+    if (!s.serves(bytes, need) && !s.ensure(bytes, need)) return -1;
+
+    if (in_kind == Mem::kDevice || out_kind == Mem::kDevice) {
+        // order after the work already queued on the null stream (a synchronous copy's rule)
+        hipError_t e = hipEventRecord(s.order, nullptr);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s.stream, s.order, 0);
+        if (e != hipSuccess) {
+            set_error("ordering after the null stream", e);
+            return -1;
+        }
+    }
+    if (small_pageable) {
         std::memcpy(s.direct, in, bytes);
         const cf *h_in = reinterpret_cast<const cf *>(s.direct_dev);
         cf *h_out = reinterpret_cast<cf *>(s.direct_dev + kDirectBytes);
@@ -309,6 +419,59 @@ int execute_host_call(const onefft_plan *plan, const float *in, float *out, bool
         return -1;
     }
     return 0;
+}
+
+}  // namespace
+
+bool staging_prepare(int device, size_t n, bool buffers) {
+    Pool *p = pool(device);
+    if (!p) {
+        set_error("no staging pool for this device");
+        return false;
+    }
+    std::lock_guard<std::mutex> lk(p->grow);
+    const int want = slots_wanted();
+    while (p->count.load(std::memory_order_relaxed) < want)
+        if (!add_slot(*p, false)) return false;
+    if (!buffers) return true;
+    const size_t bytes = n * 2 * sizeof(float);
+    const unsigned need = needs_for_size(bytes);
+    if (need & kNeedRing) (void)CopyPool::get();  // its helper threads start now, not in a call
+    const int count = p->count.load(std::memory_order_relaxed);
+    for (int i = 0; i < count; i++) {
+        Slot &s = p->slots[i];
+        // take the slot from whichever call holds it (prepare is not a real-time call)
+        while (s.busy.exchange(true, std::memory_order_acquire)) std::this_thread::yield();
+        const bool ok = s.serves(bytes, need) || s.ensure(bytes, need);
+        s.busy.store(false, std::memory_order_release);
+        if (!ok) return false;
+    }
+    return true;
+}
+
+void staging_release_all() {
+    std::lock_guard<std::mutex> lk(g_pools_mutex);
+    for (int d = 0; d < kMaxDevices; d++) {
+        Pool *p = __atomic_load_n(&g_pools[d], __ATOMIC_RELAXED);
+        if (!p) continue;
+        std::lock_guard<std::mutex> lg(p->grow);
+        const int count = p->count.load(std::memory_order_relaxed);
+        if (!count) continue;
+        DeviceGuard g(d);
+        for (int i = 0; i < count; i++) p->slots[i].release();
+        p->count.store(0, std::memory_order_release);
+    }
+}
+
+int execute_host_call(const onefft_plan *plan, const float *in, float *out, bool inverse) {
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) {
+        set_error("no HIP device");
+        return -1;
+    }
+    Claim c(claim(device));
+    if (!c.s) return -1;
+    return transform(*c.s, plan, in, out, inverse);
 }
 
 }  // namespace onefft

@@ -155,6 +155,27 @@ int onefft_ragged_execute_per_item(const onefft_ragged_plan *plan, const float *
  * completed: 0, or -1 with onefft_last_error() set. */
 int onefft_ragged_wait(const onefft_ragged_plan *plan, struct ihipStream_t *stream);
 
+/* ---- ragged batches with no allocation ------------------------------------------------------
+ * No counterpart in the reference. For callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ *
+ * Transforms the `count` items whose descriptors are at d_items (DEVICE memory, 8-byte aligned)
+ * in one launch sequence on `stream`, with no plan and no allocation: a planner kernel builds the
+ * task queue in a fixed device buffer of the library (about 5.2 MiB per GPU, held until the
+ * process ends), then the persistent kernel of onefft_ragged_execute runs it, with results
+ * bit-identical to it. d_in and d_out (device memory, 8-byte aligned, not overlapping) hold
+ * `extent` complex values; every item must lie within them and items must not overlap.
+ * Limits: count <= 131072 items (else -1 here) and at most 262144 tasks (a 2^20-point item makes
+ * 128 of them, a single-pass group of items 1). The device writes *d_status (4-byte aligned device
+ * memory) when the work completes: 0 done, 1 the wait for the first passes timed out, 2 an item
+ * invalid (log2n > 20, direction not 0/1, or beyond `extent`; nothing transformed), 3 too many
+ * tasks (nothing transformed).
+ * The first call on a GPU uploads the library's twiddle tables (synchronously; not under graph
+ * capture); later calls only launch two kernels. Calls share the buffer per GPU: calls that overlap
+ * on one GPU race, so order them (one stream, or synchronise). Returns 0 or -1. */
+int onefft_ragged_execute_items(const onefft_ragged_item *d_items, size_t count, size_t extent, const float *d_in,
+                                float *d_out, uint32_t *d_status, struct ihipStream_t *stream);
+
 /* ---- HIP graphs: a plan execution recorded once, replayed with one launch ------------------- */
 
 typedef struct onefft_graph onefft_graph;

@@ -48,6 +48,7 @@ SIGNATURES = {
     "onefft_ragged_execute": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "onefft_ragged_execute_per_item": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "onefft_ragged_wait": (c_int, [c_void_p, c_void_p]),
+    "onefft_ragged_execute_items": (c_int, [c_void_p, c_size_t, c_size_t, c_void_p, c_void_p, c_void_p, c_void_p]),
     "onefft_graph_create_batch": (c_void_p, [c_void_p, c_void_p, c_void_p, c_size_t, c_size_t, c_size_t, c_int]),
     "onefft_graph_create_ragged": (c_void_p, [c_void_p, c_void_p, c_void_p]),
     "onefft_graph_launch": (c_int, [c_void_p, c_void_p]),

@@ -23,7 +23,7 @@ LIB = os.path.join(HERE, "libonefft_amd.so")
 STREAM_SOURCE = "stream_copy.hip"
 STREAM_LIB = os.path.join(HERE, "libonefft_stream.so")
 SOURCES = ["runtime.hip", "alloc.hip", "staging.hip", "ragged.hip", "graph.hip", "tuning.hip", "k_batch.hip", "k_batch_tw.hip", "k_batch_tw_large.hip", "k_pass.hip", "k_pass_halves.hip",
-           "k_ragged.hip"]
+           "k_ragged.hip", "ragged_static.hip"]
 ARCH = "gfx950"
 
 

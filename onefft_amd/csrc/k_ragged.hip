@@ -220,6 +220,11 @@ __global__ __launch_bounds__(kRaggedWG) void fft_ragged_kernel(RaggedArgs a) {
     __shared__ cf lds[kLds];
     __shared__ uint32_t claimed;
     const uint32_t t = threadIdx.x;
+    uint32_t ntasks = a.ntasks, nphase1 = a.nphase1;
+    if (a.counts) {  // a queue built on the device: its counts are in memory (uniform loads)
+        ntasks = __builtin_amdgcn_readfirstlane(a.counts[0]);
+        nphase1 = __builtin_amdgcn_readfirstlane(a.counts[1]);
+    }
     uint32_t mine = 0;       // phase-1 tasks this workgroup ran
     bool published = false;  // its phase-1 results are published
     for (;;) {
@@ -227,20 +232,20 @@ __global__ __launch_bounds__(kRaggedWG) void fft_ragged_kernel(RaggedArgs a) {
         __syncthreads();
         const uint32_t id = __builtin_amdgcn_readfirstlane(claimed);  // uniform: task fields go to SGPRs
         __syncthreads();  // everyone has read `claimed` before it is overwritten
-        if (id >= a.nphase1 && !published) {
+        if (id >= nphase1 && !published) {
             // phase 1 is claimed out: every wave writes its stores back, then one atomic publishes
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             __syncthreads();
             if (t == 0) __hip_atomic_fetch_add(a.head + 1, mine, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             published = true;
-            if (id < a.ntasks) {
+            if (id < ntasks) {
                 // a second-pass task: wait until every phase-1 task is published
                 if (t == 0) {
                     for (uint32_t spins = 0;
-                         __hip_atomic_load(a.head + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < a.nphase1;
+                         __hip_atomic_load(a.head + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < nphase1;
                          spins++) {
                         if (spins >= a.spin_limit) {
-                            __hip_atomic_store(a.head + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            __hip_atomic_store(a.error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                             break;
                         }
                         __builtin_amdgcn_s_sleep(8);
@@ -253,7 +258,7 @@ __global__ __launch_bounds__(kRaggedWG) void fft_ragged_kernel(RaggedArgs a) {
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             }
         }
-        if (id >= a.ntasks) break;
+        if (id >= ntasks) break;
         const RaggedTask tk = a.tasks[id];
         const uint32_t kind = tk.word & 3u, log2n = (tk.word >> 2) & 31u;
         const bool inv = (tk.word >> 7) & 1u;
@@ -266,13 +271,14 @@ __global__ __launch_bounds__(kRaggedWG) void fft_ragged_kernel(RaggedArgs a) {
         } else {
             pass_tiles(a, tk, log2n, inv, kind == kRaggedFirst, lds, t);
         }
-        if (id < a.nphase1) mine++;
+        if (id < nphase1) mine++;
         __syncthreads();  // LDS is free for the next task
     }
 }
 
-__global__ __launch_bounds__(64) void fft_ragged_reset(uint32_t *head) {
-    if (threadIdx.x < 3) head[threadIdx.x] = 0;
+__global__ __launch_bounds__(64) void fft_ragged_reset(uint32_t *head, uint32_t *error) {
+    if (threadIdx.x < 2) head[threadIdx.x] = 0;
+    if (threadIdx.x == 2) *error = 0;
 }
 
 uint32_t ragged_resident_grid() {
@@ -288,12 +294,17 @@ uint32_t ragged_resident_grid() {
 }
 
 hipError_t launch_ragged(const RaggedArgs &a, uint32_t grid, hipStream_t stream) {
-    hipLaunchKernelGGL(fft_ragged_reset, dim3(1), dim3(64), 0, stream, a.head);
+    hipLaunchKernelGGL(fft_ragged_reset, dim3(1), dim3(64), 0, stream, a.head, a.error);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (a.ntasks == 0) return hipSuccess;
     if (grid == 0) grid = ragged_resident_grid();
     if (grid > a.ntasks) grid = a.ntasks;
+    return launch_ragged_kernel(a, grid, stream);
+}
+
+hipError_t launch_ragged_kernel(const RaggedArgs &a, uint32_t grid, hipStream_t stream) {
+    if (grid == 0) grid = ragged_resident_grid();
     hipLaunchKernelGGL(fft_ragged_kernel, dim3(grid), dim3(kRaggedWG), 0, stream, a);
     return hipGetLastError();
 }

@@ -128,8 +128,13 @@ constexpr int kRaggedSplitLog2 = 19;     // default split size (ONEFFT_RAGGED_SP
 
 enum RaggedKind : uint32_t { kRaggedSingle = 0, kRaggedFirst = 1, kRaggedSecond = 2, kRaggedWhole = 3 };
 
+// A task's first word: kind | log2n << 2 | inverse << 7 | (tiles - 1) << 8.
+__host__ __device__ constexpr uint32_t ragged_word(uint32_t kind, uint32_t log2n, uint32_t inverse, uint32_t tiles = 1) {
+    return kind | (log2n << 2) | (inverse << 7) | ((tiles - 1) << 8);
+}
+
 struct RaggedTask {
-    uint32_t word;  // kind | log2n << 2 | inverse << 7 | (tiles - 1) << 8
+    uint32_t word;  // ragged_word(kind, log2n, inverse, tiles)
     uint32_t item;  // single: first index into the sorted item list; others: the item's index
     uint32_t arg;   // single: number of transforms; pass runs: the first tile; whole: 0
     uint32_t tw64;  // single: the FP64 stage bases of n, as an offset in the table blob, read per
@@ -152,7 +157,10 @@ struct RaggedArgs {
     cf *out;
     const RaggedTask *tasks;
     const uint64_t *item_off;  // complex offset of each item, in task order
-    uint32_t *head;            // [0] next task to claim, [1] phase-1 tasks published, [2] error flag
+    uint32_t *head;            // [0] next task to claim, [1] phase-1 tasks published
+    uint32_t *error;           // set to 1 if the bounded wait for phase 1 gives up
+    const uint32_t *counts;    // when set, ntasks and nphase1 are read from counts[0], counts[1]
+                               // (a queue built on the device)
     const cd *tw64;            // FP64 stage bases and two-level FP64 tables of w_n
     uint32_t ntasks;
     uint32_t nphase1;          // tasks [0, nphase1) are phase 1
@@ -161,9 +169,16 @@ struct RaggedArgs {
     RaggedSize size[kRaggedMaxLog2 + 1];
 };
 
-// Zeroes the queue counters (with a kernel, so the pair can be captured in a graph), then runs the
-// persistent kernel on `grid` workgroups (0: as many as fit the device at once).
+// Zeroes the queue counters and the error flag (with a kernel, so the pair can be captured in a
+// graph), then runs the persistent kernel on `grid` workgroups (0: as many as fit the device at
+// once).
 hipError_t launch_ragged(const RaggedArgs &a, uint32_t grid, hipStream_t stream);
+// The persistent kernel alone, on `grid` workgroups, for a queue whose counters were zeroed and
+// whose counts (a.counts) were written by earlier work on `stream`.
+hipError_t launch_ragged_kernel(const RaggedArgs &a, uint32_t grid, hipStream_t stream);
+
+// Tasks per single-pass task of 2^log2n points (whole transforms, one column each).
+constexpr uint32_t ragged_per_task(int log2n) { return (uint32_t)kRaggedWG >> (log2n - batch_log2p(log2n)); }
 // Workgroups of the persistent kernel that fit the device at once.
 uint32_t ragged_resident_grid();
 

@@ -31,14 +31,29 @@ struct onefft_ragged_plan {
 };
 
 namespace onefft {
-namespace {
 
-uint32_t per_task(unsigned log2n) {
-    const unsigned lp = (unsigned)batch_log2p((int)log2n);
-    return (uint32_t)kRaggedWG >> (log2n - lp);
+RaggedChoice ragged_choice() {
+    RaggedChoice c;
+    // two-pass items below the split size run whole on one workgroup; larger ones are split into
+    // runs of tiles over both phases (ONEFFT_RAGGED_SPLIT: the split size's log2, for measurement)
+    c.split = (uint32_t)kRaggedSplitLog2;
+    if (const char *env = std::getenv("ONEFFT_RAGGED_SPLIT")) {
+        const int v = std::atoi(env);
+        if (v > kRaggedSingleMaxLog2 && v <= kRaggedMaxLog2 + 1) c.split = (uint32_t)v;
+    }
+    // ONEFFT_RAGGED_TILE_RUN: tiles per pass task, 1..256 (for measurement)
+    c.tile_run = kRaggedTileRun;
+    if (const char *env = std::getenv("ONEFFT_RAGGED_TILE_RUN")) {
+        const int v = std::atoi(env);
+        if (v >= 1 && v <= 256) c.tile_run = (uint32_t)v;
+    }
+    // the bounded wait for phase 1: ~3 s of 0.2 us sleeps (ONEFFT_RAGGED_SPIN_LIMIT overrides it,
+    // so tests can make the wait give up and check that the failure is reported)
+    c.spin_limit = 1u << 24;
+    if (const char *env = std::getenv("ONEFFT_RAGGED_SPIN_LIMIT")) c.spin_limit = (uint32_t)std::strtoul(env, nullptr, 10);
+    return c;
 }
 
-}  // namespace
 }  // namespace onefft
 
 using namespace onefft;
@@ -135,22 +150,8 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
     for (size_t k = 0; k < count; k++) item_off[k] = items[order[k]].offset;
 
     std::vector<RaggedTask> tasks;
-    auto word = [](uint32_t kind, uint32_t log2n, uint32_t inv, uint32_t tiles = 1) {
-        return kind | (log2n << 2) | (inv << 7) | ((tiles - 1) << 8);
-    };
-    // two-pass items below the split size run whole on one workgroup; larger ones are split into
-    // runs of tiles over both phases (ONEFFT_RAGGED_SPLIT: the split size's log2, for measurement)
-    uint32_t split = (uint32_t)kRaggedSplitLog2;
-    if (const char *env = std::getenv("ONEFFT_RAGGED_SPLIT")) {
-        const int v = std::atoi(env);
-        if (v > kRaggedSingleMaxLog2 && v <= kRaggedMaxLog2 + 1) split = (uint32_t)v;
-    }
-    // ONEFFT_RAGGED_TILE_RUN: tiles per pass task, 1..256 (for measurement)
-    uint32_t tile_run = kRaggedTileRun;
-    if (const char *env = std::getenv("ONEFFT_RAGGED_TILE_RUN")) {
-        const int v = std::atoi(env);
-        if (v >= 1 && v <= 256) tile_run = (uint32_t)v;
-    }
+    const RaggedChoice choice = ragged_choice();
+    const uint32_t split = choice.split, tile_run = choice.tile_run;
     auto pass_tasks = [&](uint32_t kind) {
         for (size_t k = 0; k < nbig; k++) {
             const onefft_ragged_item &it = items[order[k]];
@@ -158,7 +159,7 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
             const uint32_t tiles = 1u << (it.log2n - kRaggedTaskLog2);  // tiles per pass
             for (uint32_t t0 = 0; t0 < tiles; t0 += tile_run) {
                 const uint32_t run = tiles - t0 < tile_run ? tiles - t0 : tile_run;
-                tasks.push_back({word(kind, it.log2n, it.inverse, run), (uint32_t)k, t0, 0});
+                tasks.push_back({ragged_word(kind, it.log2n, it.inverse, run), (uint32_t)k, t0, 0});
             }
         }
     };
@@ -168,16 +169,16 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
         const onefft_ragged_item &it = items[order[k]];
         if (it.log2n < split)
             tasks.push_back(
-                {word(kRaggedWhole, it.log2n, it.inverse, 1u << (it.log2n - kRaggedTaskLog2)), (uint32_t)k, 0, 0});
+                {ragged_word(kRaggedWhole, it.log2n, it.inverse, 1u << (it.log2n - kRaggedTaskLog2)), (uint32_t)k, 0, 0});
     }
     pass_tasks(kRaggedFirst);
     for (size_t k = nbig; k < count;) {
         const onefft_ragged_item &it = items[order[k]];
         size_t e = k;
-        const uint32_t cap = per_task(it.log2n);
+        const uint32_t cap = ragged_per_task((int)it.log2n);
         while (e < count && e - k < cap && items[order[e]].log2n == it.log2n && items[order[e]].inverse == it.inverse)
             e++;
-        tasks.push_back({word(kRaggedSingle, it.log2n, it.inverse), (uint32_t)k, (uint32_t)(e - k), a.size[it.log2n].tw64});
+        tasks.push_back({ragged_word(kRaggedSingle, it.log2n, it.inverse), (uint32_t)k, (uint32_t)(e - k), a.size[it.log2n].tw64});
         k = e;
     }
     const size_t nphase1 = tasks.size();
@@ -216,14 +217,13 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
     a.tasks = reinterpret_cast<const RaggedTask *>(d + o_tasks);
     a.item_off = reinterpret_cast<const uint64_t *>(d + o_off);
     a.head = reinterpret_cast<uint32_t *>(d + o_head);
+    a.error = a.head + 2;
+    a.counts = nullptr;
     a.tw64 = reinterpret_cast<const cd *>(d + o_tw64);
     a.ntasks = (uint32_t)tasks.size();
     a.nphase1 = (uint32_t)nphase1;
     a.nitems = (uint32_t)count;
-    // the bounded wait for phase 1: ~3 s of 0.2 us sleeps (ONEFFT_RAGGED_SPIN_LIMIT overrides it,
-    // so tests can make the wait give up and check that the failure is reported)
-    a.spin_limit = 1u << 24;
-    if (const char *env = std::getenv("ONEFFT_RAGGED_SPIN_LIMIT")) a.spin_limit = (uint32_t)std::strtoul(env, nullptr, 10);
+    a.spin_limit = choice.spin_limit;
     plan->grid = ragged_resident_grid();
     return plan;
 }
@@ -315,9 +315,9 @@ extern "C" int onefft_ragged_wait(const onefft_ragged_plan *plan, struct ihipStr
         set_error("ragged execution", e);
         return -1;
     }
-    // the kernel's bounded wait for phase 1 sets head[2] if it ever gives up
+    // the kernel's bounded wait for phase 1 sets the error flag if it ever gives up
     uint32_t flag = 0;
-    e = hipMemcpy(&flag, plan->args.head + 2, sizeof(flag), hipMemcpyDeviceToHost);
+    e = hipMemcpy(&flag, plan->args.error, sizeof(flag), hipMemcpyDeviceToHost);
     if (e != hipSuccess) {
         set_error("ragged status read", e);
         return -1;

@@ -70,6 +70,13 @@ struct DeviceGuard {
     DeviceGuard &operator=(const DeviceGuard &) = delete;
 };
 
+// The ragged executions' choices: the split size (log2; two-pass items from it up are spread over
+// tile runs, smaller ones run whole), tiles per pass task, and the bounded wait's sleep count.
+struct RaggedChoice {
+    uint32_t split, tile_run, spin_limit;
+};
+RaggedChoice ragged_choice();
+
 // The device a plan or a ragged plan belongs to.
 int plan_device(const onefft_plan *plan);
 int ragged_plan_device(const onefft_ragged_plan *plan);

@@ -44,9 +44,14 @@ def parse():
     ap.add_argument("--single-transform", type=int, default=None, metavar="LOG2N",
                     help="instead of the benchmark: 20 forward transforms of one 2^LOG2N-point input (random, "
                          "resident in HBM) and one device copy of the same bytes, for traces and counter runs")
+    ap.add_argument("--batch", type=int, default=1,
+                    help="with --single-transform: transforms per launch (contiguous), default 1")
     ap.add_argument("--ragged-only", action="store_true",
                     help="instead of the benchmark: 3 one-launch executions of the 4 GiB ragged prefix (seed-3 items, "
                          "random input resident in HBM), for counter runs")
+    ap.add_argument("--ragged-breakdown", action="store_true",
+                    help="instead of the benchmark: per size class of the 4 GiB ragged prefix, one ragged launch "
+                         "of that class's items against one standalone batched launch of as many transforms")
     ap.add_argument("--report", action="store_true",
                     help="add a per-configuration report: errors against the reference CPU transform and the "
                          "FP64 oracle, and the reference's CPU times (several minutes of host work)")
@@ -118,6 +123,58 @@ def variants_ab(torch, F, sizes):
         os.environ.pop("ONEFFT_BATCH_VARIANT", None)
     else:
         os.environ["ONEFFT_BATCH_VARIANT"] = old
+
+
+def batched_equivalent(torch, F, stream, items, x, y, reps=3):
+    """The ragged items' size histogram run as standalone batched launches (one per size, forward,
+    contiguous transforms in x and y): per log2 n the item count and milliseconds per launch."""
+    sp = stream.cuda_stream
+    out = {}
+    for l in sorted({it[1] for it in items}):
+        count = sum(1 for it in items if it[1] == l)
+        n = 1 << l
+        p = F.plan(n)
+        assert count * n <= x.numel()
+        run = lambda: p.batch_ptr(x.data_ptr(), y.data_ptr(), count, n, n, False, sp)
+        run()
+        out[l] = (count, time_ms(torch, stream, run, reps))
+    return out
+
+
+def ragged_breakdown(torch, F):
+    """--ragged-breakdown: the 4 GiB ragged prefix, per size class."""
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    items = F.ragged_items(3, 200000, max_bytes=4 << 30)
+    extent = items[-1][0] + (1 << items[-1][1])
+    x = torch.randn(extent, dtype=torch.complex64, device="cuda")
+    y = torch.empty_like(x)
+    rp = F.RaggedPlan(items)
+    one = lambda: rp.execute(x.data_ptr(), y.data_ptr(), sp)
+    one()
+    rp.wait(sp)
+    full = time_ms(torch, stream, one, 5)
+    rp.close()
+    batched = batched_equivalent(torch, F, stream, items, x, y)
+    rows = []
+    for l, (count, bms) in batched.items():
+        sub, off = [], 0
+        for _, ll, inv in items:
+            if ll == l:
+                sub.append((off, l, inv))
+                off += 1 << l
+        p = F.RaggedPlan(sub)
+        run = lambda: p.execute(x.data_ptr(), y.data_ptr(), sp)
+        run()
+        p.wait(sp)
+        rms = time_ms(torch, stream, run, 3)
+        p.close()
+        rows.append({"log2n": l, "items": count, "ragged_ms": round(rms, 4), "batched_ms": round(bms, 4),
+                     "gb_s_ragged": round(16.0 * count * (1 << l) / (rms * 1e-3) / 1e9, 1)})
+        print(json.dumps(rows[-1]), flush=True)
+    print(json.dumps({"ragged_prefix_one_launch_ms": round(full, 3),
+                      "sum_of_class_launches_ms": round(sum(r["ragged_ms"] for r in rows), 3),
+                      "sum_of_batched_launches_ms": round(sum(r["batched_ms"] for r in rows), 3)}), flush=True)
 
 
 def extra_workloads(torch, F, stream):
@@ -196,12 +253,16 @@ def extra_workloads(torch, F, stream):
     per()
     torch.cuda.synchronize()
     t_per = time_ms(torch, stream, per, 2)
+    t_batched = sum(ms for _, ms in batched_equivalent(torch, F, stream, items, x, y).values())
     flops = sum(5.0 * (1 << l) * l for _, l, _ in items)
     out["ragged"] = {"items": len(items), "gib_in": round(extent * 8 / 2**30, 3), "tasks": rp.tasks,
                      "one_launch_ms": round(t_one, 3), "graph_ms": round(t_graph, 3), "per_item_ms": round(t_per, 3),
                      "one_launch_gflops": round(flops / (t_one * 1e-3) / 1e9, 1),
                      "alg_gb_s": round(16.0 * extent / (t_one * 1e-3) / 1e9, 1),
                      "speedup_vs_per_item": round(t_per / t_one, 2),
+                     "standalone_batched_ms": round(t_batched, 3),
+                     "standalone_batched_note": "the same size histogram as one batched forward launch per "
+                                                "size (17 launches), summed",
                      "interpretation": "seed-3 items (log2 n ~ U{4..20}, 50/50 directions), longest prefix "
                                        "with input <= 4 GiB, packed back to back"}
     g.close()
@@ -560,16 +621,19 @@ def main():
         return
     if args.single_transform is not None:
         n = 1 << args.single_transform
-        x = torch.randn(n, dtype=torch.complex64, device="cuda")
+        x = torch.randn(args.batch * n, dtype=torch.complex64, device="cuda")
         y = torch.empty_like(x)
         p = F.Plan(n)
         sp = torch.cuda.current_stream().cuda_stream
         for _ in range(20):  # the first few run while the clocks leave idle
-            p.batch_ptr(x.data_ptr(), y.data_ptr(), 1, n, n, False, sp)
+            p.batch_ptr(x.data_ptr(), y.data_ptr(), args.batch, n, n, False, sp)
         z = torch.empty_like(x)
-        z.copy_(x)  # a plain copy of known size: 8 n bytes read, 8 n bytes written
+        z.copy_(x)  # a plain copy of known size: 8 n batch bytes read, as many written
         torch.cuda.synchronize()
-        print(json.dumps({"single_transform": args.single_transform, "launches": 20}), flush=True)
+        print(json.dumps({"single_transform": args.single_transform, "batch": args.batch, "launches": 20}), flush=True)
+        return
+    if args.ragged_breakdown:
+        ragged_breakdown(torch, F)
         return
     if args.ragged_only:
         items = F.ragged_items(3, 200000, max_bytes=4 << 30)

@@ -175,9 +175,25 @@ __device__ __forceinline__ void single_body(const IO &io, bool live, const cf *_
 //
 // Columns tile*C .. tile*C + C-1 of the pass described by `a` (see k_pass.hip for the index
 // scheme), on the transform at src/dst. Thread t of the C * 2^LOG2NF / P threads handles column
-// t % C. lds holds C * pass_span(LOG2NF) complex values (kPadded) or floats (kHalves: the stage
-// exchanges move real and imaginary parts in turn, as in single_finish).
-__host__ __device__ constexpr uint32_t pass_span(int log2nf) { return lds_span(1u << log2nf) | 1; }
+// t % C. lds holds C * pass_span(LOG2NF, C, LAYOUT) complex values (kPadded) or floats (kHalves:
+// the stage exchanges move real and imaginary parts in turn, as in single_finish).
+//
+// The columns' LDS regions lie pass_span values apart. A wave's 64 lanes cover C columns x 64/C
+// rows, and a row step moves one 4-byte bank (halves) or two (padded): the regions must start on
+// banks that keep the C columns' lanes apart. With 32 banks of 4 bytes (the LDS of gfx950 as its
+// bank-conflict counter sees it) that is a column step of an odd multiple of 32/C banks when
+// C < 32 (an odd number of values otherwise). A bank model of every pass shape, checked against
+// SQ_LDS_BANK_CONFLICT (640 and 0 cycles per wave measured and modelled for the 2^11 x 4 and
+// 2^8 x 32 tiles with the old odd spans), finds this choice optimal for every allowed shape; with
+// few columns it removes most conflicts (2^11 x 4: 640 -> 64 cycles per wave, 2^12 x 4: 640 -> 0).
+__host__ __device__ constexpr uint32_t pass_span(int log2nf, int cols, int layout) {
+    const uint32_t need = lds_span(1u << log2nf);
+    const uint32_t width = layout == kHalves ? 1 : 2;  // banks per value
+    const uint32_t unit = 32u / (uint32_t)cols;        // banks between columns' regions
+    if (unit <= width) return need | 1;
+    const uint32_t u = unit / width, r = need % (2 * u);  // want span = u (mod 2u), in values
+    return need + (u >= r ? u - r : 3 * u - r);
+}
 
 template <int LOG2NF, int C, bool FIRST, bool SWAP_IN, bool SWAP_OUT, int LAYOUT = kPadded>
 __device__ __forceinline__ void pass_body(const PassArgs &a, const cf *__restrict__ src, cf *__restrict__ dst,
@@ -188,7 +204,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, const cf *__restric
     constexpr int P = 1 << LOG2P;
     constexpr uint32_t T = NF / P;  // threads per column
     constexpr int S = stage_count(LOG2NF, LOG2P);
-    constexpr uint32_t SPAN = pass_span(LOG2NF);  // odd: the C columns start on different banks
+    constexpr uint32_t SPAN = pass_span(LOG2NF, C, LAYOUT);  // the columns start on distinct banks
 
     const uint32_t c = t % C;
     const uint32_t j = t / C;

@@ -193,6 +193,17 @@ __device__ __forceinline__ uint32_t xcd_contiguous(uint32_t b, uint32_t count) {
     const uint32_t q = count / kXcds, r = count % kXcds, x = b % kXcds, i = b / kXcds;
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
 }
+// The same, with each XCD's range cut into `strips` equal parts that the XCD runs side by side
+// (its k-th workgroup takes the (k / strips)-th tile of strip k mod strips), so that the tiles in
+// flight on one XCD lie in several parts of the range; strips 1, or a range it does not divide,
+// gives xcd_contiguous.
+__device__ __forceinline__ uint32_t xcd_strips(uint32_t b, uint32_t count, uint32_t strips) {
+    const uint32_t q = count / kXcds, r = count % kXcds, x = b % kXcds;
+    uint32_t i = b / kXcds;
+    const uint32_t len = q + (x < r ? 1 : 0);
+    if (strips > 1 && len % strips == 0) i = (i % strips) * (len / strips) + i / strips;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
 
 // ---- global access --------------------------------------------------------------------------
 //

@@ -28,11 +28,12 @@ template <int LOG2NF, int C, bool FIRST, bool SWAP_IN, bool SWAP_OUT, int LAYOUT
 __global__ __launch_bounds__(pass_wg(LOG2NF, C)) void fft_pass_kernel(PassArgs a) {
     constexpr bool USES_LDS = stage_count(LOG2NF, pass_log2p(LOG2NF)) > 1;
     // C columns of pass_span values: complex values, or floats with the halves layout
-    constexpr uint32_t VALUES = USES_LDS ? (LAYOUT == kHalves ? (C * pass_span(LOG2NF) + 1) / 2 : C * pass_span(LOG2NF)) : 1;
+    constexpr uint32_t SPAN = pass_span(LOG2NF, C, LAYOUT);
+    constexpr uint32_t VALUES = USES_LDS ? (LAYOUT == kHalves ? (C * SPAN + 1) / 2 : C * SPAN) : 1;
     __shared__ cf lds[VALUES];
     // tiles in XCD-contiguous order over the whole grid (tile-major within a transform)
     const uint32_t tiles = gridDim.x;
-    const uint32_t w = xcd_contiguous(blockIdx.x + blockIdx.y * tiles, tiles * gridDim.y);
+    const uint32_t w = xcd_strips(blockIdx.x + blockIdx.y * tiles, tiles * gridDim.y, a.strips);
     const uint32_t tile = w % tiles, which = w / tiles;
     pass_body<LOG2NF, C, FIRST, SWAP_IN, SWAP_OUT, LAYOUT>(a, a.in + (uint64_t)which * a.in_dist,
                                                            a.out + (uint64_t)which * a.out_dist, tile, lds, threadIdx.x);

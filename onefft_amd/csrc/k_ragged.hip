@@ -25,11 +25,11 @@ namespace onefft {
 namespace {
 
 // LDS of one task: the largest of the single-pass layouts (kRaggedTaskPoints points) and the pass
-// layouts (C = kRaggedTaskPoints / N_f columns of pass_span(N_f) values).
+// layouts (C = kRaggedTaskPoints / N_f columns of pass_span(N_f, C) values).
 constexpr uint32_t ragged_lds_values() {
     uint32_t m = kRaggedTaskPoints / 16 * lds_span(16);
     for (int l = 6; l <= 10; l++) {
-        uint32_t v = (kRaggedTaskPoints >> l) * pass_span(l);
+        uint32_t v = (kRaggedTaskPoints >> l) * pass_span(l, (int)(kRaggedTaskPoints >> l), kPadded);
         if (v > m) m = v;
     }
     return m;
@@ -194,7 +194,7 @@ __device__ void pass_tile(const RaggedArgs &a, const RaggedSize &z, const cf *in
         io.swap_out = inv;
         io.low = col;
     }
-    column_fft(io, lf, a.tw64 + tw64, lds + c * pass_span((int)lf), j);
+    column_fft(io, lf, a.tw64 + tw64, lds + c * pass_span((int)lf, (int)C, kPadded), j);
 }
 
 // a run of tiles of one pass of a two-pass item

@@ -96,6 +96,7 @@ struct PassArgs {
     uint32_t dig_log2[kMaxPasses];   // log2 N_i of each digit
     uint32_t dig_shift[kMaxPasses];  // log2 of each digit's weight in the row index
     uint32_t columns;     // n / N_f
+    uint32_t strips;      // tile order: XCD-contiguous ranges run as this many strips side by side
 };
 
 // One pass of a multi-pass transform: first = reads the input (out of place), else in place on

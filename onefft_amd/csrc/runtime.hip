@@ -297,6 +297,10 @@ extern "C" onefft_plan *onefft_plan_create(size_t n) {
                 a.log2m = log2m;
                 a.tw_shift = log2n - log2m - (unsigned)lf;
                 a.columns = (uint32_t)(n >> lf);
+                // ONEFFT_PASS_STRIPS: strips of the input pass's tile order (for measurement)
+                a.strips = 1;
+                if (const char *env = std::getenv("ONEFFT_PASS_STRIPS"))
+                    if (i == 0) a.strips = (uint32_t)std::strtoul(env, nullptr, 10);
                 if (i == 0) {
                     a.ndig = (uint32_t)(p - 1);
                     for (int d = 0; d < p - 1; d++) {  // digit j_{d+1}

@@ -77,14 +77,14 @@ def time_ms(torch, stream, fn, reps):
 
 
 def variants_ab(torch, F, sizes):
-    """Interleaved A/B of the single-pass kernel variants (ONEFFT_BATCH_VARIANT = "nt,layout,tw64", tw64 a
-    level 0..3)."""
+    """Interleaved A/B of the single-pass kernel variants (ONEFFT_BATCH_VARIANT = "nt,layout,tw64[,xlane]",
+    tw64 a level 0..3, xlane the 32-point kernel's DPP pair exchange)."""
     import statistics
 
     variants = {"padded": "0,0,0", "padded_nt": "1,0,0", "swizzled": "0,1,0", "swizzled_nt": "1,1,0",
                 "padded_tw64": "0,0,1", "padded_nt_tw64": "1,0,1", "padded_nt_tw64_ns16": "1,0,2",
                 "padded_nt_tw64_all": "1,0,3", "halves_nt": "1,2,0", "halves_nt_tw64_ns16": "1,2,2",
-                "halves_nt_tw64_all": "1,2,3"}
+                "halves_nt_tw64_all": "1,2,3", "padded_nt_xlane": "1,0,0,1", "padded_xlane": "0,0,0,1"}
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
     old = os.environ.get("ONEFFT_BATCH_VARIANT")

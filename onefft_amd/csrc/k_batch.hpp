@@ -61,7 +61,15 @@ inline void launch_variant(bool inverse, const BatchArgs &a, dim3 grid, hipStrea
 // values past the end.
 constexpr uint32_t kSmallBlock = 4096;
 
-template <int LOG2N, bool INV, bool NT>
+// The value the neighbouring lane (lane ^ 1) holds: a DPP quad_perm [1, 0, 3, 2] move, a VALU
+// operation with no LDS traffic and no barrier.
+__device__ __forceinline__ float from_pair_lane(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0xB1, 0xF, 0xF, false));
+}
+
+// XLANE (n = 32 only): the two threads of a transform exchange their halves by DPP lane swaps
+// instead of through LDS.
+template <int LOG2N, bool INV, bool NT, bool XLANE = false>
 __global__ __launch_bounds__(256) void fft_small_kernel(const cf *__restrict__ in, cf *__restrict__ out,
                                                          uint64_t total) {
     constexpr int N = 1 << LOG2N;
@@ -89,6 +97,23 @@ __global__ __launch_bounds__(256) void fft_small_kernel(const cf *__restrict__ i
         static_for<0, 16>([&](auto k) { v[k] = maybe_conj<INV>(lds[lds_pad(c32 + 2u * decltype(k)::value + j)]); });
         dft<16>(v);
         if (j) static_for<1, 16>([&](auto m) { v[m] = twc<32, decltype(m)::value>(v[m]); });
+        if constexpr (XLANE) {
+            // thread 0 holds E[0..15], thread 1 O[0..15]: each sends the half the other combines
+            // (E[8..15] to thread 1, O[0..7] to thread 0) and forms outputs m, m + 16 for
+            // m = 8 j .. 8 j + 7 from E[m] +- O[m]
+            __syncthreads();  // every thread has read its inputs
+            static_for<0, 8>([&](auto i) {
+                constexpr int k = decltype(i)::value;
+                const cf send = j ? v[k] : v[k + 8], own = j ? v[k + 8] : v[k];
+                const cf got = mk(from_pair_lane(send.r), from_pair_lane(send.i));
+                const cf e = j ? got : own, o = j ? own : got;
+                const uint32_t m = 8u * j + k;
+                lds[lds_pad(c32 + m)] = maybe_conj<INV>(add(e, o));
+                lds[lds_pad(c32 + 16u + m)] = maybe_conj<INV>(sub(e, o));
+            });
+            __syncthreads();
+            static_for<0, 16>([&](auto r) { v[r] = lds[lds_pad(t + decltype(r)::value * 256u)]; });
+        } else {
         __syncthreads();  // every thread has read its inputs
         static_for<0, 16>([&](auto m) { lds[lds_pad(c32 + 16u * j + decltype(m)::value)] = v[m]; });
         __syncthreads();
@@ -101,6 +126,7 @@ __global__ __launch_bounds__(256) void fft_small_kernel(const cf *__restrict__ i
         });
         __syncthreads();
         static_for<0, 16>([&](auto r) { v[r] = lds[lds_pad(t + decltype(r)::value * 256u)]; });
+        }
     } else if constexpr (LOG2N > 0) {
         static_for<0, 16>([&](auto r) { lds[lds_pad(t + decltype(r)::value * 256u)] = v[r]; });
         __syncthreads();

@@ -37,12 +37,14 @@ struct BatchArgs {
 // Kernel variant of the single-pass transform: nontemporal global access, the LDS layout
 // (fft_common.hpp: kPadded / kSwizzled / kHalves, the last from 2^kHalvesMinLog2 points), and the
 // level of FP64-base twiddles (0..3, fft_common.hpp: stage_uses_tw64; with the padded and halves
-// layouts). The plan's tables serve every variant.
+// layouts), and for 32-point contiguous batches whether the pair exchange uses DPP lane swaps. The
+// plan's tables serve every variant.
 constexpr int kHalvesMinLog2 = 8;
 struct BatchVariant {
     bool nt = true;
     int layout = 0;
     int tw64 = 1;
+    bool xlane = false;  // 32 points: the transform's two threads exchange by DPP lane swaps, not LDS
 };
 
 // One launch: `batch` transforms of 2^log2n points, 0 <= log2n <= kMaxSinglePassLog2.

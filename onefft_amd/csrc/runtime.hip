@@ -112,18 +112,20 @@ int choose_split_factors(unsigned log2n, int regime, int *factors) {
 }
 
 // Single-pass kernel variant: the tuning table's choice, else the BatchVariant defaults
-// (kernels.hpp). ONEFFT_BATCH_VARIANT="nt,layout[,tw64]" (nt 0/1, layout 0..2, tw64 the FP64-base
-// level 0..3, 1 when omitted) overrides the table, for measurement.
+// (kernels.hpp). ONEFFT_BATCH_VARIANT="nt,layout[,tw64[,xlane]]" (nt 0/1, layout 0..2, tw64 the
+// FP64-base level 0..3, 1 when omitted, xlane 0/1 the DPP pair exchange of 32-point batches)
+// overrides the table, for measurement.
 onefft::BatchVariant choose_variant(unsigned log2n, int regime) {
     onefft::BatchVariant v;
     const onefft::TuneEntry &t = onefft::tune_lookup(log2n, regime);
     if (t.kind == onefft::TuneEntry::kSingle) v = t.variant;
     if (const char *env = std::getenv("ONEFFT_BATCH_VARIANT")) {
-        int nt = 1, layout = 0, tw64 = 1;
-        if (std::sscanf(env, "%d,%d,%d", &nt, &layout, &tw64) >= 2) {
+        int nt = 1, layout = 0, tw64 = 1, xlane = 0;
+        if (std::sscanf(env, "%d,%d,%d,%d", &nt, &layout, &tw64, &xlane) >= 2) {
             v.nt = nt != 0;
             v.layout = layout < 0 ? 0 : layout > 2 ? 2 : layout;
             v.tw64 = tw64 < 0 ? 0 : tw64 > 3 ? 3 : tw64;
+            v.xlane = xlane != 0;
         }
     }
     return v;

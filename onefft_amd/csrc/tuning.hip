@@ -2,7 +2,7 @@
 // multi-pass split and columns per workgroup that measured fastest on this GPU.
 //
 // File: tab-separated lines "log2n  kind  params  gb_s  batch", '#' comments, where kind is
-// "single" (params "nt=0|1,layout=0..2[,tw64=0..3]") or "multi" (params
+// "single" (params "nt=0|1,layout=0..2[,tw64=0..3[,xlane=0|1]]") or "multi" (params
 // "split=a,b[,c[,d]];cols=x,y[,..][;layout=0|2]", log2 N_1 first; layout: the passes' LDS layout,
 // padded when absent). The batch the line was measured at sets its working-set regime
 // (runtime.hpp: tune_regime); a size may have a line per regime. Written by `python -m onefft_amd.tune`; read once, at the first plan creation,
@@ -62,12 +62,13 @@ void parse_line(const char *line) {
     const int regime = tune_regime(uint64_t(1) << log2n, batch);
     TuneEntry e;
     if (!std::strcmp(kind, "single")) {
-        int nt = 1, layout = 0, tw64 = 0;  // tables written before FP64 bases existed meant tw64=0
-        if (std::sscanf(params, "nt=%d,layout=%d,tw64=%d", &nt, &layout, &tw64) < 2) return;
+        int nt = 1, layout = 0, tw64 = 0, xlane = 0;  // tables written before FP64 bases existed meant tw64=0
+        if (std::sscanf(params, "nt=%d,layout=%d,tw64=%d,xlane=%d", &nt, &layout, &tw64, &xlane) < 2) return;
         e.kind = TuneEntry::kSingle;
         e.variant.nt = nt != 0;
         e.variant.layout = layout < 0 ? 0 : layout > 2 ? 2 : layout;
         e.variant.tw64 = tw64 < 0 ? 0 : tw64 > 3 ? 3 : tw64;
+        e.variant.xlane = xlane != 0;
     } else if (!std::strcmp(kind, "multi")) {
         const char *sp = std::strstr(params, "split="), *cp = std::strstr(params, "cols=");
         if (!sp || !cp) return;

@@ -6,7 +6,7 @@ tuning table the library reads at plan creation (tuning.hip).
 * sizes up to 2^14 (one LDS pass): the single-pass variants (nontemporal global access on or off,
   padded or swizzled LDS, and, where the size has such stages, FP64-base twiddles at levels 1..3:
   for the stages with Ns >= 64, >= 16, or all; from 2^8 also the halves LDS layout at every
-  level);
+  level; at 2^5 the pair exchange by DPP lane swaps);
 * larger sizes: every split n = N_1 ... N_p into 2..4 passes with factors 2^6..2^10 (4 passes only
   from 2^24; two- and three-pass splits also with 2^11 and 2^12), each with the default and the
   widest allowed columns per workgroup and with the padded or the halves LDS layout, and with the
@@ -102,8 +102,12 @@ def candidates(L: int, single: bool):
         combos += [(nt, 0, level) for level in tw64_levels(L) for nt in (1, 0)]
         if L >= HALVES_MIN_LOG2:  # the halves layout, with every level
             combos += [(nt, 2, level) for level in [0] + tw64_levels(L) for nt in (1, 0)]
-        return [dict(kind="single", params=f"nt={nt},layout={lay},tw64={t64}",
-                     env={"ONEFFT_BATCH_VARIANT": f"{nt},{lay},{t64}"}) for nt, lay, t64 in combos]
+        out = [dict(kind="single", params=f"nt={nt},layout={lay},tw64={t64}",
+                    env={"ONEFFT_BATCH_VARIANT": f"{nt},{lay},{t64}"}) for nt, lay, t64 in combos]
+        if L == 5:  # 32 points: the pair exchange by DPP lane swaps instead of LDS
+            out += [dict(kind="single", params=f"nt={nt},layout=0,tw64=0,xlane=1",
+                         env={"ONEFFT_BATCH_VARIANT": f"{nt},0,0,1"}) for nt in (1, 0)]
+        return out
     out = []
     for s in splits(L):
         # the passes' LDS layout (padded, or halves: kernels.hpp), with the default and the widest

@@ -143,12 +143,14 @@ def test_every_single_pass_variant(monkeypatch):
         batch = 3
         x = O.normal(60 + L, n * batch)
         d = torch.from_numpy(x).cuda()
-        variants = [(nt, lay, 0) for nt in (0, 1) for lay in (0, 1)]
-        variants += [(nt, 0, lv) for lv in tw64_levels(L) for nt in (0, 1)]
+        variants = [(nt, lay, 0, 0) for nt in (0, 1) for lay in (0, 1)]
+        variants += [(nt, 0, lv, 0) for lv in tw64_levels(L) for nt in (0, 1)]
         if L >= 8:  # the halves layout (kernels.hpp: kHalvesMinLog2)
-            variants += [(nt, 2, lv) for lv in [0] + tw64_levels(L) for nt in (0, 1)]
-        for nt, lay, lv in variants:
-            monkeypatch.setenv("ONEFFT_BATCH_VARIANT", f"{nt},{lay},{lv}")
+            variants += [(nt, 2, lv, 0) for lv in [0] + tw64_levels(L) for nt in (0, 1)]
+        if L == 5:  # the DPP pair exchange of the 32-point kernel
+            variants += [(nt, 0, 0, 1) for nt in (0, 1)]
+        for nt, lay, lv, xl in variants:
+            monkeypatch.setenv("ONEFFT_BATCH_VARIANT", f"{nt},{lay},{lv},{xl}")
             p = F.Plan(n)
             for inv in (False, True):
                 o = torch.empty_like(d)
@@ -158,8 +160,36 @@ def test_every_single_pass_variant(monkeypatch):
                 for b in range(batch):
                     seg = slice(2 * n * b, 2 * n * (b + 1))
                     e = O.rel_rms(got[seg], O.fft_f64(x[seg], n, inv))
-                    assert e < tol_fp64(L), (L, nt, lay, lv, inv, b, e)
+                    assert e < tol_fp64(L), (L, nt, lay, lv, xl, inv, b, e)
             p.close()
+
+
+def test_dpp_pair_exchange_bitwise_equals_lds(monkeypatch):
+    """The 32-point kernel's pair exchange by DPP lane swaps gives the LDS exchange's bits (the same
+    additions), over whole 4096-value blocks and a partial last block, both directions."""
+    import torch
+
+    import onefft_amd as F
+
+    s = torch.cuda.current_stream().cuda_stream
+    n, batch = 32, 128 * 5 + 37  # 5 whole blocks of 128 transforms and a partial one
+    x = torch.randn(batch * n, dtype=torch.complex64, device="cuda")
+    outs = {}
+    for xl in (0, 1):
+        monkeypatch.setenv("ONEFFT_BATCH_VARIANT", f"1,0,0,{xl}")
+        p = F.Plan(n)
+        for inv in (False, True):
+            o = torch.full_like(x, complex(7.0, 7.0))
+            p.batch_ptr(x.data_ptr(), o.data_ptr(), batch, n, n, inv, s)
+            torch.cuda.synchronize()
+            outs[xl, inv] = o
+        p.close()
+    for inv in (False, True):
+        assert torch.equal(outs[0, inv].view(torch.float32), outs[1, inv].view(torch.float32)), inv
+        ref = torch.fft.fft(x.view(batch, n).to(torch.complex128)) if not inv else \
+            torch.fft.ifft(x.view(batch, n).to(torch.complex128)) * n
+        err = (torch.linalg.vector_norm(outs[1, inv].view(batch, n) - ref) / torch.linalg.vector_norm(ref)).item()
+        assert err < 4e-7 + 1e-7 * 5, err
 
 
 @pytest.mark.parametrize("split,cols,batch", [("10,11", "8,4", 3), ("11,11", "4,4", 2), ("11,12", "4,2", 1),

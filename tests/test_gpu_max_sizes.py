@@ -78,6 +78,11 @@ def test_multipass_batch_across_launch_chunks(monkeypatch, inverse, wide):
     in_dist, out_dist = n + 8, n + 24
     print(f"chunked batch n=2^{L} x{batch} inverse={inverse} wide={wide}", flush=True)
     if wide:
+        # the library accepts these column counts (else it would silently fall back to defaults)
+        from onefft_amd.tune import pass_cols_ok
+
+        assert pass_cols_ok(7, 64, 2) and pass_cols_ok(8, 64, 2)
+        assert not pass_cols_ok(8, 64, 0), "64 columns of 2^8 points need the halves layout's tiles"
         monkeypatch.setenv("ONEFFT_PASS_SPLIT", "7,8")
         monkeypatch.setenv("ONEFFT_PASS_COLS", "64,64")
         monkeypatch.setenv("ONEFFT_PASS_LAYOUT", "2")

@@ -195,6 +195,80 @@ def test_generator_statistics():
     assert abs(g.mean()) < 0.01 and abs(g.var() - 1) < 0.02 and abs((g ** 3).mean()) < 0.05
 
 
+def _xorshift_step(state):
+    """One step of the oracle's xorshift generator: (next state, output), read from the function's
+    own state update (oracle_xorshift_fill writes the advanced state back)."""
+    import ctypes
+
+    st = ctypes.c_uint32(state)
+    out = np.empty(1, np.float32)
+    O.lib().oracle_xorshift_fill(ctypes.byref(st), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), 1)
+    return st.value, float(out[0])
+
+
+def _gf2_matmul(a, b):
+    """32 x 32 bit matrices over GF(2), as lists of column words: (a b) e_i = a (b e_i)."""
+    def apply(m, v):
+        r, i = 0, 0
+        while v:
+            if v & 1:
+                r ^= m[i]
+            v >>= 1
+            i += 1
+        return r
+    return [apply(a, col) for col in b]
+
+
+def _gf2_pow(m, e):
+    result = [1 << i for i in range(32)]
+    while e:
+        if e & 1:
+            result = _gf2_matmul(m, result)
+        m = _gf2_matmul(m, m)
+        e >>= 1
+    return result
+
+
+def test_xorshift_structure():
+    """oracle_xorshift_fill (the reference tests' xorshift32, tests/test_onefft.c:57-61) pinned by
+    properties, not by a second copy of its formula: its state transition T is linear over GF(2);
+    T has the full period 2^32 - 1 (its bit matrix has that multiplicative order, the defining
+    property of Marsaglia's shift triple 13, 17, 5), so no state but 0 repeats early; each output is
+    the new state read as a signed 32-bit integer times 2^-31, exactly where that is representable;
+    and filling in pieces continues the same stream."""
+    identity = [1 << i for i in range(32)]
+    cols = [_xorshift_step(1 << i)[0] for i in range(32)]
+    rng = np.random.default_rng(5)
+    for a, b in rng.integers(0, 2**32, (200, 2), dtype=np.uint64).tolist():
+        assert _xorshift_step(a ^ b)[0] == _xorshift_step(a)[0] ^ _xorshift_step(b)[0]
+        # the bit matrix reproduces T
+        v, r, i = a, 0, 0
+        while v:
+            if v & 1:
+                r ^= cols[i]
+            v >>= 1
+            i += 1
+        assert r == _xorshift_step(a)[0]
+    order = 2**32 - 1  # = 3 * 5 * 17 * 257 * 65537
+    assert _gf2_pow(cols, order) == identity
+    for p in (3, 5, 17, 257, 65537):
+        assert _gf2_pow(cols, order // p) != identity, p
+    # outputs: state / 2^31, exact for |state| < 2^24 as a signed integer
+    for s0 in rng.integers(0, 2**32, 20000, dtype=np.uint64).tolist():
+        s1, out = _xorshift_step(s0)
+        signed = s1 - 2**32 if s1 >= 2**31 else s1
+        assert -1.0 <= out < 1.0
+        if abs(signed) < 2**24:
+            assert out * 2.0**31 == signed
+        else:
+            assert abs(out * 2.0**31 - signed) <= abs(signed) * 2.0**-24
+    # the default stream, in pieces and at once
+    whole = O.xorshift(1000)
+    assert np.array_equal(whole[:2], O.xorshift(2))
+    stats = whole.astype(np.float64)
+    assert abs(stats.mean()) < 0.05 and abs(stats.var() - 1 / 3) < 0.03
+
+
 # ---- planted mistakes must be caught -------------------------------------------------------------
 
 def test_planted_mistakes_fail_the_checks(golden):

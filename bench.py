@@ -600,7 +600,9 @@ def cpu_baseline(budget_s: float):
         "kind": "reference",
         "isa": O.ref_implementation(),
         "sample": f"{sample} of the {BATCH} transforms of n={N} (re/im N(0,1), seed 1), {threads} threads "
-                  f"sharing one plan, min over {samples} samples (bench/bench.c method)",
+                  f"sharing one plan, min over {samples} samples (bench/bench.c method); working set "
+                  f"{8 * N * sample >> 20} MiB in + {8 * N * sample >> 20} MiB out, which may sit partly in "
+                  f"the host's last-level cache",
         "gb_s": round(16.0 * N * sample / per / 1e9, 2),
         "host": host_cpu(),
     }

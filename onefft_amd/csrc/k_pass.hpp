@@ -16,8 +16,11 @@
 //
 // The pass twiddles w_n^x span all n exponents. They are formed in FP64 from two small FP64
 // tables (w_n^x = w_n^(x_hi 2^b) w_n^(x_lo)) and a short FP64 recurrence per thread, then rounded
-// once to FP32: the twiddle values are those of the reference's FP64-computed, once-rounded tables
-// (src/onefft.c:112-141) without reading an n-entry table from HBM.
+// once to FP32, without reading an n-entry table from HBM. The values equal the reference's
+// FP64-computed, once-rounded twiddles (src/onefft.c:112-141) except where a component that the
+// reference holds as exactly 0 (a quarter turn) comes out as a residue of a few 1e-16, or where
+// the exact value lies within the FP64 formation error of an FP32 rounding boundary (then one
+// unit in the last place).
 #pragma once
 
 #include "fft_bodies.hpp"

@@ -115,7 +115,7 @@ int onefft_plan_forward_batch(const onefft_plan *plan, const float *d_in, float 
 int onefft_plan_inverse_batch(const onefft_plan *plan, const float *d_in, float *d_out, size_t batch,
                               size_t in_dist, size_t out_dist, struct ihipStream_t *stream);
 
-/* ---- ragged batches: many transforms of mixed sizes in one launch ---------------------------
+/* ---- ragged batches: many transforms of mixed sizes in one execution ------------------------
  * (no reference counterpart: with the reference each array is its own onefft_forward/inverse
  * call, include/onefft.h:76-77). */
 
@@ -131,16 +131,23 @@ typedef struct {
 typedef struct onefft_ragged_plan onefft_ragged_plan;
 
 /* Validates the items (sizes, directions, no two overlapping), builds the task queue and the
- * tables on the current device. NULL on invalid items or allocation failure. */
+ * tables on the current device. Items of the tuned batched size and up (the tuning table's ragged
+ * line; 2^20 as shipped) are grouped per size and direction for the multi-pass kernels, which run
+ * them at their offsets; the rest go to the persistent kernel's task queue. The plan may hold its
+ * own stream and two events (to run the persistent kernel beside the groups). NULL on invalid
+ * items or allocation failure. */
 onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_item *items, size_t count);
 void onefft_ragged_plan_destroy(onefft_ragged_plan *plan);
 
 /* Complex values the input and output buffers must hold (largest offset + size), and the number
- * of tasks one execution runs. */
+ * of tasks the persistent kernel runs in one execution (the grouped items are not counted). */
 size_t onefft_ragged_extent(const onefft_ragged_plan *plan);
 size_t onefft_ragged_task_count(const onefft_ragged_plan *plan);
 
-/* Transforms every item in one persistent-kernel launch, asynchronously on `stream`. d_in and
+/* Transforms every item, asynchronously on `stream`: the grouped items by multi-pass launches at
+ * their offsets, the rest by one persistent-kernel launch, either after them on `stream` or beside
+ * them on the plan's own stream (forked from and joined back to `stream` by events, so work queued
+ * on `stream` afterwards, a graph capture of the call included, sees every item done). d_in and
  * d_out are device memory, 8-byte aligned, each holding onefft_ragged_extent() complex values,
  * not overlapping (-1 if they do); d_in is not modified. A ragged plan runs one execution at a
  * time: executions of one plan must be ordered (same stream or synchronised). Returns 0 or -1. */
@@ -162,8 +169,9 @@ int onefft_ragged_wait(const onefft_ragged_plan *plan, struct ihipStream_t *stre
  * Transforms the `count` items whose descriptors are at d_items (DEVICE memory, 8-byte aligned)
  * in one launch sequence on `stream`, with no plan and no allocation: a planner kernel builds the
  * task queue in a fixed device buffer of the library (about 5.2 MiB per GPU, held until the
- * process ends), then the persistent kernel of onefft_ragged_execute runs it, with results
- * bit-identical to it. d_in and d_out (device memory, 8-byte aligned, not overlapping) hold
+ * process ends), then the persistent kernel of onefft_ragged_execute runs every item, with results
+ * bit-identical to onefft_ragged_execute of a plan whose items all go to that kernel
+ * (ONEFFT_RAGGED_BATCHED_MIN=21) and within FP32 rounding of the grouped execution. d_in and d_out (device memory, 8-byte aligned, not overlapping) hold
  * `extent` complex values; every item must lie within them and items must not overlap.
  * Limits: count <= 131072 items (else -1 here) and at most 262144 tasks (a 2^20-point item makes
  * 128 of them, a single-pass group of items 1). The device writes *d_status (4-byte aligned device

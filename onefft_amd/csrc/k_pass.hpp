@@ -38,8 +38,9 @@ __global__ __launch_bounds__(pass_wg(LOG2NF, C)) void fft_pass_kernel(PassArgs a
     const uint32_t tiles = gridDim.x;
     const uint32_t w = xcd_strips(blockIdx.x + blockIdx.y * tiles, tiles * gridDim.y, a.strips);
     const uint32_t tile = w % tiles, which = w / tiles;
-    pass_body<LOG2NF, C, FIRST, SWAP_IN, SWAP_OUT, LAYOUT>(a, a.in + (uint64_t)which * a.in_dist,
-                                                           a.out + (uint64_t)which * a.out_dist, tile, lds, threadIdx.x);
+    const uint64_t in_at = a.offsets ? a.offsets[which] : (uint64_t)which * a.in_dist;
+    const uint64_t out_at = a.offsets ? a.offsets[which] : (uint64_t)which * a.out_dist;
+    pass_body<LOG2NF, C, FIRST, SWAP_IN, SWAP_OUT, LAYOUT>(a, a.in + in_at, a.out + out_at, tile, lds, threadIdx.x);
 }
 
 template <int LOG2NF, int C, int LAYOUT>

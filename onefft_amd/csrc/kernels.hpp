@@ -89,6 +89,8 @@ struct PassArgs {
     const cd *tw_hi;      // w_n^(i 2^lo_bits), i < n / 2^lo_bits (FP64)
     uint64_t in_dist;     // complex values between transforms of a batch
     uint64_t out_dist;
+    const uint64_t *offsets;  // if set, transform b starts at offsets[b] complex values in both
+                              // buffers (a ragged plan's items of one size), not at b * *_dist
     uint32_t lo_bits;
     uint32_t log2n;
     uint32_t log2m;       // low-index extent M_f of this pass (0 for the first pass)
@@ -128,6 +130,9 @@ constexpr int kRaggedSingleMaxLog2 = kRaggedTaskLog2;
 constexpr int kRaggedWG = kRaggedTaskPoints / 16;
 constexpr uint32_t kRaggedTileRun = 2;   // tiles per pass task (at most 256)
 constexpr int kRaggedSplitLog2 = 19;     // default split size (ONEFFT_RAGGED_SPLIT overrides)
+// default smallest size that a ragged plan runs as offset-batched multi-pass groups instead of in
+// the persistent kernel (the tuning table's ragged line or ONEFFT_RAGGED_BATCHED_MIN overrides)
+constexpr int kRaggedBatchedMinLog2 = 16;
 
 enum RaggedKind : uint32_t { kRaggedSingle = 0, kRaggedFirst = 1, kRaggedSecond = 2, kRaggedWhole = 3 };
 

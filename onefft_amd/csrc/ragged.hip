@@ -1,5 +1,8 @@
-// Ragged-batch plans: the host builds the task queue and the tables once; an execution is one
-// reset kernel plus one persistent kernel (k_ragged.hip).
+// Ragged-batch plans: the host builds the task queue and the tables once. An execution is, for
+// the items of the batched size and up (2^15..2^20 points; the threshold is tuned), one group per
+// size and direction run by the multi-pass kernels at the items' offsets with that size's tuned
+// split; for the rest, one reset kernel plus one persistent kernel (k_ragged.hip) over a device
+// task queue.
 //
 // No counterpart in the reference, whose API transforms one array per call
 // (include/onefft.h:61-77); onefft_ragged_execute_per_item runs the same items that way (one
@@ -20,32 +23,59 @@
 using onefft::cd;
 using onefft::cf;
 
+// The items of one size and direction that run as one offset-batched multi-pass execution.
+struct RaggedGroup {
+    const onefft_plan *plan;   // the size's plan (one of sized_plans)
+    bool inverse;
+    uint32_t count;
+    const uint64_t *offsets;   // device memory: the items' offsets, `count` entries
+};
+
 struct onefft_ragged_plan {
     int device = 0;
     size_t count = 0;
     uint64_t extent = 0;  // complex values the buffers must hold
     void *dmem = nullptr;
-    onefft::RaggedArgs args{};
+    onefft::RaggedArgs args{};  // the persistent kernel's items (args.ntasks == 0: none)
     uint32_t grid = 0;
-    std::vector<onefft_ragged_item> items;  // as given, for the per-item comparison
+    uint32_t order = 0;              // onefft::RaggedOrder
+    hipStream_t side = nullptr;      // kConcurrent: the persistent kernel's stream
+    hipEvent_t fork = nullptr, join = nullptr;
+    std::vector<RaggedGroup> groups;          // largest size first
+    std::vector<onefft_plan *> sized_plans;   // owned: one per group size
+    std::vector<onefft_ragged_item> items;    // as given, for the per-item comparison
 };
 
 namespace onefft {
 
 RaggedChoice ragged_choice() {
     RaggedChoice c;
+    const RaggedTune &t = ragged_tune();
+    // items of 2^batched_min points and more run as offset-batched multi-pass groups
+    // (kRaggedMaxLog2 + 1: none); the tuning table's choice, ONEFFT_RAGGED_BATCHED_MIN overrides it
+    c.batched_min = t.set ? t.batched_min : (uint32_t)kRaggedBatchedMinLog2;
+    if (const char *env = std::getenv("ONEFFT_RAGGED_BATCHED_MIN")) {
+        const int v = std::atoi(env);
+        if (v > kMaxSinglePassLog2 && v <= kRaggedMaxLog2 + 1) c.batched_min = (uint32_t)v;
+    }
     // two-pass items below the split size run whole on one workgroup; larger ones are split into
-    // runs of tiles over both phases (ONEFFT_RAGGED_SPLIT: the split size's log2, for measurement)
-    c.split = (uint32_t)kRaggedSplitLog2;
+    // runs of tiles over both phases (ONEFFT_RAGGED_SPLIT: the split size's log2)
+    c.split = t.set ? t.split : (uint32_t)kRaggedSplitLog2;
     if (const char *env = std::getenv("ONEFFT_RAGGED_SPLIT")) {
         const int v = std::atoi(env);
         if (v > kRaggedSingleMaxLog2 && v <= kRaggedMaxLog2 + 1) c.split = (uint32_t)v;
     }
-    // ONEFFT_RAGGED_TILE_RUN: tiles per pass task, 1..256 (for measurement)
-    c.tile_run = kRaggedTileRun;
+    // ONEFFT_RAGGED_TILE_RUN: tiles per pass task, 1..256
+    c.tile_run = t.set ? t.tile_run : kRaggedTileRun;
     if (const char *env = std::getenv("ONEFFT_RAGGED_TILE_RUN")) {
         const int v = std::atoi(env);
         if (v >= 1 && v <= 256) c.tile_run = (uint32_t)v;
+    }
+    // ONEFFT_RAGGED_ORDER: 0 groups first, 1 persistent kernel first, 2 side by side
+    c.order = t.set ? t.order : (uint32_t)kGroupsFirst;
+    if (const char *env = std::getenv("ONEFFT_RAGGED_ORDER")) {
+        const int v = std::atoi(env);
+        if (v >= 0 && v <= (int)kConcurrent) c.order = (uint32_t)v;
     }
     // the bounded wait for phase 1: ~3 s of 0.2 us sleeps (ONEFFT_RAGGED_SPIN_LIMIT overrides it,
     // so tests can make the wait give up and check that the failure is reported)
@@ -98,11 +128,33 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
     plan->count = count;
     plan->extent = extent;
     plan->items.assign(items, items + count);
+    const RaggedChoice choice = ragged_choice();
+    const uint32_t split = choice.split, tile_run = choice.tile_run;
+    auto batched = [&](const onefft_ragged_item &it) { return it.log2n >= choice.batched_min; };
 
-    // tables, all FP64: the stage bases of every size used as a single pass or as a sub-transform,
-    // and the two-level table of w_n of every two-pass size
+    // the batched groups: one per size and direction, largest size first
+    std::vector<std::vector<uint64_t>> group_off;
+    std::vector<std::pair<uint32_t, bool>> group_key;
+    for (int l = kRaggedMaxLog2; l > kMaxSinglePassLog2; l--) {
+        if ((uint32_t)l < choice.batched_min) break;
+        for (int inv = 0; inv < 2; inv++) {
+            std::vector<uint64_t> offs;
+            for (size_t i = 0; i < count; i++)
+                if (items[i].log2n == (uint32_t)l && items[i].inverse == (uint32_t)inv) offs.push_back(items[i].offset);
+            if (offs.empty()) continue;
+            group_off.push_back(std::move(offs));
+            group_key.push_back({(uint32_t)l, inv != 0});
+        }
+    }
+
+    // the persistent kernel's items: tables, all FP64: the stage bases of every size used as a
+    // single pass or as a sub-transform, and the two-level table of w_n of every two-pass size
+    std::vector<uint32_t> order;
+    for (size_t i = 0; i < count; i++)
+        if (!batched(items[i])) order.push_back((uint32_t)i);
+    const size_t nper = order.size();
     bool used[kRaggedMaxLog2 + 1] = {};
-    for (size_t i = 0; i < count; i++) used[items[i].log2n] = true;
+    for (uint32_t i : order) used[items[i].log2n] = true;
     std::vector<cd> tw64(1);
     int32_t stage64_at[kRaggedMaxLog2 + 1];
     for (auto &v : stage64_at) v = -1;
@@ -138,20 +190,16 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
 
     // items in task order: two-pass items (largest first), then single-pass items grouped by size
     // (largest first) and direction
-    std::vector<uint32_t> order(count);
-    for (size_t i = 0; i < count; i++) order[i] = (uint32_t)i;
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
         if (items[x].log2n != items[y].log2n) return items[x].log2n > items[y].log2n;
         return items[x].inverse < items[y].inverse;
     });
     size_t nbig = 0;
-    while (nbig < count && items[order[nbig]].log2n > (uint32_t)kRaggedSingleMaxLog2) nbig++;
-    std::vector<uint64_t> item_off(count ? count : 1);
-    for (size_t k = 0; k < count; k++) item_off[k] = items[order[k]].offset;
+    while (nbig < nper && items[order[nbig]].log2n > (uint32_t)kRaggedSingleMaxLog2) nbig++;
+    std::vector<uint64_t> item_off(nper ? nper : 1);
+    for (size_t k = 0; k < nper; k++) item_off[k] = items[order[k]].offset;
 
     std::vector<RaggedTask> tasks;
-    const RaggedChoice choice = ragged_choice();
-    const uint32_t split = choice.split, tile_run = choice.tile_run;
     auto pass_tasks = [&](uint32_t kind) {
         for (size_t k = 0; k < nbig; k++) {
             const onefft_ragged_item &it = items[order[k]];
@@ -172,11 +220,11 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
                 {ragged_word(kRaggedWhole, it.log2n, it.inverse, 1u << (it.log2n - kRaggedTaskLog2)), (uint32_t)k, 0, 0});
     }
     pass_tasks(kRaggedFirst);
-    for (size_t k = nbig; k < count;) {
+    for (size_t k = nbig; k < nper;) {
         const onefft_ragged_item &it = items[order[k]];
         size_t e = k;
         const uint32_t cap = ragged_per_task((int)it.log2n);
-        while (e < count && e - k < cap && items[order[e]].log2n == it.log2n && items[order[e]].inverse == it.inverse)
+        while (e < nper && e - k < cap && items[order[e]].log2n == it.log2n && items[order[e]].inverse == it.inverse)
             e++;
         tasks.push_back({ragged_word(kRaggedSingle, it.log2n, it.inverse), (uint32_t)k, (uint32_t)(e - k), a.size[it.log2n].tw64});
         k = e;
@@ -190,17 +238,25 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
         return nullptr;
     }
 
-    // one device allocation: tasks | item offsets | queue counters | tw64
+    // one device allocation: tasks | item offsets | queue counters | tw64 | group offsets
     auto up = [](size_t v) { return (v + 255) / 256 * 256; };
     const size_t o_tasks = 0;
     const size_t o_off = up(o_tasks + tasks.size() * sizeof(RaggedTask));
     const size_t o_head = up(o_off + item_off.size() * sizeof(uint64_t));
     const size_t o_tw64 = up(o_head + 4 * sizeof(uint32_t));
-    const size_t bytes = up(o_tw64 + tw64.size() * sizeof(cd));
+    size_t end = up(o_tw64 + tw64.size() * sizeof(cd));
+    std::vector<size_t> o_group(group_off.size());
+    for (size_t g = 0; g < group_off.size(); g++) {
+        o_group[g] = end;
+        end = up(end + group_off[g].size() * sizeof(uint64_t));
+    }
+    const size_t bytes = end;
     std::vector<unsigned char> image(bytes, 0);
     std::memcpy(image.data() + o_tasks, tasks.data(), tasks.size() * sizeof(RaggedTask));
     std::memcpy(image.data() + o_off, item_off.data(), item_off.size() * sizeof(uint64_t));
     std::memcpy(image.data() + o_tw64, tw64.data(), tw64.size() * sizeof(cd));
+    for (size_t g = 0; g < group_off.size(); g++)
+        std::memcpy(image.data() + o_group[g], group_off[g].data(), group_off[g].size() * sizeof(uint64_t));
     if (dev_malloc(&plan->dmem, bytes) != hipSuccess) {
         set_error("ragged plan allocation failed");
         delete plan;
@@ -208,9 +264,8 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
     }
     if (hipMemcpy(plan->dmem, image.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
         (void)hipGetLastError();
-        (void)hipFree(plan->dmem);
         set_error("ragged plan upload failed");
-        delete plan;
+        onefft_ragged_plan_destroy(plan);
         return nullptr;
     }
     unsigned char *d = static_cast<unsigned char *>(plan->dmem);
@@ -222,23 +277,56 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
     a.tw64 = reinterpret_cast<const cd *>(d + o_tw64);
     a.ntasks = (uint32_t)tasks.size();
     a.nphase1 = (uint32_t)nphase1;
-    a.nitems = (uint32_t)count;
+    a.nitems = (uint32_t)nper;
     a.spin_limit = choice.spin_limit;
     plan->grid = ragged_resident_grid();
+    plan->order = choice.order;
+    // side by side needs both parts, a stream for the persistent kernel and two events
+    if (plan->order == kConcurrent && a.ntasks && !group_off.empty()) {
+        if (stream_create(&plan->side) != hipSuccess || event_create(&plan->fork) != hipSuccess ||
+            event_create(&plan->join) != hipSuccess) {
+            set_error("ragged plan stream or event creation failed");
+            onefft_ragged_plan_destroy(plan);
+            return nullptr;
+        }
+    }
+
+    // the groups' plans (tables only: no staging), one per size
+    for (size_t g = 0; g < group_off.size(); g++) {
+        const uint32_t l = group_key[g].first;
+        onefft_plan *sp = nullptr;
+        for (onefft_plan *q : plan->sized_plans)
+            if (onefft_plan_size(q) == (size_t(1) << l)) sp = q;
+        if (!sp) {
+            sp = plan_build(size_t(1) << l, false);
+            if (!sp) {
+                onefft_ragged_plan_destroy(plan);
+                return nullptr;
+            }
+            plan->sized_plans.push_back(sp);
+        }
+        plan->groups.push_back({sp, group_key[g].second, (uint32_t)group_off[g].size(),
+                                reinterpret_cast<const uint64_t *>(d + o_group[g])});
+    }
     return plan;
 }
 
 extern "C" void onefft_ragged_plan_destroy(onefft_ragged_plan *plan) {
     if (!plan) return;
-    if (plan->dmem) {
+    for (onefft_plan *p : plan->sized_plans) onefft_plan_destroy(p);
+    if (plan->dmem || plan->side || plan->fork || plan->join) {
         DeviceGuard g(plan->device);
-        (void)hipFree(plan->dmem);
+        if (plan->side) (void)hipStreamDestroy(plan->side);
+        if (plan->fork) (void)hipEventDestroy(plan->fork);
+        if (plan->join) (void)hipEventDestroy(plan->join);
+        if (plan->dmem) (void)hipFree(plan->dmem);
     }
     delete plan;
 }
 
 extern "C" size_t onefft_ragged_extent(const onefft_ragged_plan *plan) { return plan ? (size_t)plan->extent : 0; }
 
+// the persistent kernel's tasks (the batched groups' items are not counted)
 extern "C" size_t onefft_ragged_task_count(const onefft_ragged_plan *plan) { return plan ? plan->args.ntasks : 0; }
 
 int onefft::ragged_plan_device(const onefft_ragged_plan *plan) { return plan->device; }
@@ -270,15 +358,47 @@ extern "C" int onefft_ragged_execute(const onefft_ragged_plan *plan, const float
         set_error("cannot select the plan's device");
         return -1;
     }
-    RaggedArgs a = plan->args;
-    a.in = reinterpret_cast<const cf *>(d_in);
-    a.out = reinterpret_cast<cf *>(d_out);
-    hipError_t e = launch_ragged(a, plan->grid, stream);
-    if (e != hipSuccess) {
-        set_error("ragged launch", e);
-        return -1;
+    const cf *in = reinterpret_cast<const cf *>(d_in);
+    cf *out = reinterpret_cast<cf *>(d_out);
+    // the batched groups, largest first
+    auto groups = [&]() -> int {
+        for (const RaggedGroup &g : plan->groups)
+            if (execute_device(g.plan, in, out, g.count, 0, 0, g.inverse, stream, g.offsets) != 0) return -1;
+        return 0;
+    };
+    // the persistent kernel over the other items
+    auto persistent = [&](hipStream_t s) -> int {
+        if (plan->args.ntasks == 0) return 0;
+        RaggedArgs a = plan->args;
+        a.in = in;
+        a.out = out;
+        hipError_t e = launch_ragged(a, plan->grid, s);
+        if (e != hipSuccess) {
+            set_error("ragged launch", e);
+            return -1;
+        }
+        return 0;
+    };
+    if (plan->side) {
+        // fork: the side stream starts after what the caller queued; join: the caller's stream
+        // continues after the side stream's kernels (this also holds under stream capture)
+        hipError_t e = hipEventRecord(plan->fork, stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(plan->side, plan->fork, 0);
+        if (e != hipSuccess) {
+            set_error("ragged fork", e);
+            return -1;
+        }
+        const int rc = groups() | persistent(plan->side);
+        e = hipEventRecord(plan->join, plan->side);
+        if (e == hipSuccess) e = hipStreamWaitEvent(stream, plan->join, 0);
+        if (e != hipSuccess) {
+            set_error("ragged join", e);
+            return -1;
+        }
+        return rc;
     }
-    return 0;
+    if (plan->order == kPersistentFirst) return persistent(stream) != 0 ? -1 : groups();
+    return groups() != 0 ? -1 : persistent(stream);
 }
 
 extern "C" int onefft_ragged_execute_per_item(const onefft_ragged_plan *plan, const float *d_in, float *d_out,

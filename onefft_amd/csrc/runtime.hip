@@ -198,9 +198,13 @@ DeviceGuard::~DeviceGuard() {
 
 // Launches the transform of `batch` arrays on `stream`; data pointers are device memory.
 int execute_device(const onefft_plan *plan, const cf *d_in, cf *d_out, uint64_t batch, uint64_t in_dist,
-                   uint64_t out_dist, bool inverse, hipStream_t stream) {
+                   uint64_t out_dist, bool inverse, hipStream_t stream, const uint64_t *d_offsets) {
     if (batch == 0) return 0;
     const int regime = tune_regime(plan->n, batch);
+    if (d_offsets && !plan->multi) {
+        set_error("transforms at listed offsets need a multi-pass plan");
+        return -1;
+    }
     if (!plan->multi) {
         BatchArgs a{d_in, d_out, plan->tw_single, plan->tw64_single, in_dist, out_dist, batch};
         hipError_t e = launch_batch((int)plan->log2n, inverse, a, plan->variant[regime], stream);
@@ -216,10 +220,16 @@ int execute_device(const onefft_plan *plan, const cf *d_in, cf *d_out, uint64_t 
         for (int i = 0; i < cfg.passes; i++) {
             PassArgs a = cfg.pass[i].args;
             const bool first = i == 0, last = i == cfg.passes - 1;
-            a.in = first ? d_in + done * in_dist : d_out + done * out_dist;
-            a.out = d_out + done * out_dist;
-            a.in_dist = first ? in_dist : out_dist;
-            a.out_dist = out_dist;
+            if (d_offsets) {
+                a.in = first ? d_in : d_out;
+                a.out = d_out;
+                a.offsets = d_offsets + done;
+            } else {
+                a.in = first ? d_in + done * in_dist : d_out + done * out_dist;
+                a.out = d_out + done * out_dist;
+                a.in_dist = first ? in_dist : out_dist;
+                a.out_dist = out_dist;
+            }
             hipError_t e = launch_pass(cfg.pass[i].log2nf, cfg.pass[i].cols, cfg.layout, first, first && inverse,
                                        last && inverse, a, chunk, stream);
             if (e != hipSuccess) {
@@ -236,7 +246,7 @@ int execute_device(const onefft_plan *plan, const cf *d_in, cf *d_out, uint64_t 
 
 using namespace onefft;
 
-extern "C" onefft_plan *onefft_plan_create(size_t n) {
+onefft_plan *onefft::plan_build(size_t n, bool staging) {
     unsigned log2n;
     if (!size_log2(n, &log2n)) return nullptr;
     onefft_plan *plan = new (std::nothrow) onefft_plan;
@@ -342,13 +352,15 @@ extern "C" onefft_plan *onefft_plan_create(size_t n) {
             }
         }
     }
-    if (!staging_prepare(plan->device, n, log2n <= kPlanStagingMaxLog2)) {
+    if (staging && !staging_prepare(plan->device, n, log2n <= kPlanStagingMaxLog2)) {
         (void)hipFree(plan->dmem);
         delete plan;
         return nullptr;
     }
     return plan;
 }
+
+extern "C" onefft_plan *onefft_plan_create(size_t n) { return plan_build(n, true); }
 
 extern "C" void onefft_plan_destroy(onefft_plan *plan) {
     if (!plan) return;

@@ -20,9 +20,15 @@ bool size_log2(size_t n, unsigned *log2n);
 void set_error(const char *what, hipError_t e);
 void set_error(const char *what);
 
-// Launch `batch` transforms on device memory, asynchronously on `stream`.
+// Launch `batch` transforms on device memory, asynchronously on `stream`. With d_offsets (device
+// memory, `batch` entries; multi-pass plans only), transform b is at d_offsets[b] complex values
+// in both buffers and the distances are ignored.
 int execute_device(const onefft_plan *plan, const cf *d_in, cf *d_out, uint64_t batch, uint64_t in_dist,
-                   uint64_t out_dist, bool inverse, hipStream_t stream);
+                   uint64_t out_dist, bool inverse, hipStream_t stream, const uint64_t *d_offsets = nullptr);
+
+// onefft_plan_create without its side effect on the staging pool (staging = false): the plans
+// that ragged plans hold for their batched groups. Freed by onefft_plan_destroy.
+onefft_plan *plan_build(size_t n, bool staging);
 
 // Sizes the staging pool of `device` (staging.hip): creates its slots (ONEFFT_STAGING_SLOTS, each
 // with a stream and events) and, with `buffers`, gives every slot what a host-pointer call of n
@@ -70,12 +76,28 @@ struct DeviceGuard {
     DeviceGuard &operator=(const DeviceGuard &) = delete;
 };
 
-// The ragged executions' choices: the split size (log2; two-pass items from it up are spread over
-// tile runs, smaller ones run whole), tiles per pass task, and the bounded wait's sleep count.
+// The ragged executions' choices: the smallest size run as offset-batched multi-pass groups
+// (log2; kRaggedMaxLog2 + 1 for none), the split size (log2; two-pass items of the persistent
+// kernel from it up are spread over tile runs, smaller ones run whole), tiles per pass task, the
+// order of the groups and the persistent kernel (RaggedOrder), and the bounded wait's sleep
+// count. The allocation-free entry point uses split, tile run and wait only (its persistent
+// kernel runs every item).
+enum RaggedOrder : uint32_t {
+    kGroupsFirst = 0,      // the groups, then the persistent kernel, on the caller's stream
+    kPersistentFirst = 1,  // the persistent kernel, then the groups
+    kConcurrent = 2,       // the persistent kernel on the plan's own stream beside the groups
+};
 struct RaggedChoice {
-    uint32_t split, tile_run, spin_limit;
+    uint32_t batched_min, split, tile_run, order, spin_limit;
 };
 RaggedChoice ragged_choice();
+
+// The tuning table's ragged line (tuning.hip), if it has one.
+struct RaggedTune {
+    bool set = false;
+    uint32_t batched_min = 0, split = 0, tile_run = 0, order = 0;
+};
+const RaggedTune &ragged_tune();
 
 // The device a plan or a ragged plan belongs to.
 int plan_device(const onefft_plan *plan);

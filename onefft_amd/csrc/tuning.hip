@@ -2,9 +2,11 @@
 // multi-pass split and columns per workgroup that measured fastest on this GPU.
 //
 // File: tab-separated lines "log2n  kind  params  gb_s  batch", '#' comments, where kind is
-// "single" (params "nt=0|1,layout=0..2[,tw64=0..3[,xlane=0|1]]") or "multi" (params
+// "single" (params "nt=0|1,layout=0..2[,tw64=0..3[,xlane=0|1]]"), "multi" (params
 // "split=a,b[,c[,d]];cols=x,y[,..][;layout=0|2]", log2 N_1 first; layout: the passes' LDS layout,
-// padded when absent). The batch the line was measured at sets its working-set regime
+// padded when absent) or "ragged" (params "batched_min=b;split=s;tile_run=t[;order=o]": ragged
+// plans' choices, one line, log2n the largest item size of the workload it was tuned on, batch its
+// item count; order 0 groups first, 1 persistent kernel first, 2 side by side). The batch the line was measured at sets its working-set regime
 // (runtime.hpp: tune_regime); a size may have a line per regime. Written by `python -m onefft_amd.tune`; read once, at the first plan creation,
 // from $ONEFFT_TUNE_FILE or tune_gfx950.tsv next to the library. Plan creation stays O(n) with no
 // measuring, as with the reference's planner (its tuning is compiled in, src/onefft.c:652-697).
@@ -23,6 +25,7 @@ namespace onefft {
 namespace {
 
 TuneEntry g_table[kTuneMaxLog2 + 1][2];
+RaggedTune g_ragged;
 std::once_flag g_once;
 
 void load_table();
@@ -59,6 +62,19 @@ void parse_line(const char *line) {
     char kind[16] = {0}, params[128] = {0};
     if (std::sscanf(line, "%d %15s %127s %lf %llu", &log2n, kind, params, &gbs, &batch) != 5) return;
     if (log2n < 0 || log2n > kTuneMaxLog2 || batch == 0) return;
+    if (!std::strcmp(kind, "ragged")) {
+        unsigned b = 0, s = 0, t = 0, o = 0;
+        if (std::sscanf(params, "batched_min=%u;split=%u;tile_run=%u;order=%u", &b, &s, &t, &o) < 3) return;
+        if (b <= (unsigned)kMaxSinglePassLog2 || b > (unsigned)kRaggedMaxLog2 + 1) return;
+        if (s <= (unsigned)kRaggedSingleMaxLog2 || s > (unsigned)kRaggedMaxLog2 + 1) return;
+        if (t < 1 || t > 256 || o > kConcurrent) return;
+        g_ragged.set = true;
+        g_ragged.batched_min = b;
+        g_ragged.split = s;
+        g_ragged.tile_run = t;
+        g_ragged.order = o;
+        return;
+    }
     const int regime = tune_regime(uint64_t(1) << log2n, batch);
     TuneEntry e;
     if (!std::strcmp(kind, "single")) {
@@ -106,6 +122,11 @@ void load_table() {
 }
 
 }  // namespace
+
+const RaggedTune &ragged_tune() {
+    std::call_once(g_once, load_table);
+    return g_ragged;
+}
 
 const TuneEntry &tune_lookup(unsigned log2n, int regime) {
     std::call_once(g_once, load_table);

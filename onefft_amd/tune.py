@@ -2,6 +2,7 @@
 tuning table the library reads at plan creation (tuning.hip).
 
     python -m onefft_amd.tune [--min 0] [--max 30] [--out onefft_amd/tune_gfx950.tsv]
+    python -m onefft_amd.tune --ragged [--out onefft_amd/tune_gfx950.tsv]
 
 * sizes up to 2^14 (one LDS pass): the single-pass variants (nontemporal global access on or off,
   padded or swizzled LDS, and, where the size has such stages, FP64-base twiddles at levels 1..3:
@@ -24,7 +25,17 @@ fastest of 5 samples of 10 back-to-back launches (single launches timed alone va
 the 2-5 % that separates the best variants). All candidates of a size give results within FP32 rounding of each other; the
 tuner checks each against the first candidate (rel-L2 <= 3e-7 log2 n). The fastest is written.
 The table is read once per process, so this module sets ONEFFT_TUNE_FILE to a missing file before
-the first plan: the candidates are measured without any table.
+the first plan: the candidates are measured without any table. Lines of sizes not measured in a
+run are kept from the existing table.
+
+Ragged plans (--ragged, run after the sizes in a process of its own, because it measures under the
+table just written: the batched groups use its multi-pass choices): the workload is the 4 GiB
+prefix of the seed-3 ragged stream (log2 n uniform on 4..20, directions 50/50); the candidates are
+the smallest size run as offset-batched multi-pass groups (2^15..2^20, or none), and, where the
+persistent kernel keeps items of 2^18 points or more, the split size and the tiles per pass task,
+and, where both parts exist, their order: groups first, persistent kernel first, or side by side on
+two streams (ONEFFT_RAGGED_BATCHED_MIN / _SPLIT / _TILE_RUN / _ORDER). Each execution is timed like a size's candidates
+and checked against the first candidate's output; the fastest is written as the table's ragged line.
 """
 from __future__ import annotations
 
@@ -159,14 +170,86 @@ def tune_size(F, torch, timed, L: int, batch: int, single: bool) -> str:
     return f"{L}\t{best[1]}\t{best[2]}\t{best[0]:.0f}\t{batch}"
 
 
+RAGGED_KEYS = ("ONEFFT_RAGGED_BATCHED_MIN", "ONEFFT_RAGGED_SPLIT", "ONEFFT_RAGGED_TILE_RUN", "ONEFFT_RAGGED_ORDER")
+
+
+def ragged_candidates():
+    """(batched_min, split, tile_run, order) choices that give distinct executions: the split size
+    and the tile run matter only where the persistent kernel keeps items of the split size (split <
+    batched_min), the order only where there are groups (batched_min <= 20); elsewhere they are
+    left at the defaults."""
+    out = []
+    for bmin in (15, 16, 17, 18, 19, 20, 21):
+        shapes = [(19, 2)] + [(split, run) for split in (18, 19, 20) if split < bmin for run in (1, 2, 4)]
+        for split, run in dict.fromkeys(shapes):
+            for order in ((0, 1, 2) if bmin <= 20 else (0,)):
+                out.append((bmin, split, run, order))
+    return out
+
+
+def tune_ragged(F, torch, stream) -> str:
+    """Measures the ragged choices on the 4 GiB seed-3 prefix; returns the table's ragged line."""
+    sp = stream.cuda_stream
+    items = F.ragged_items(3, 200000, max_bytes=4 << 30)
+    extent = items[-1][0] + (1 << items[-1][1])
+    points = sum(1 << l for _, l, _ in items)
+    x = torch.randn(extent, dtype=torch.complex64, device="cuda")
+    y = torch.empty_like(x)
+    first, results = None, []
+    for bmin, split, run, order in ragged_candidates():
+        env = dict(zip(RAGGED_KEYS, (str(bmin), str(split), str(run), str(order))))
+        os.environ.update(env)
+        try:
+            plan = F.RaggedPlan(items)
+        finally:
+            for k in env:
+                os.environ.pop(k, None)
+        execute = lambda: plan.execute(x.data_ptr(), y.data_ptr(), sp)
+        for _ in range(2):
+            execute()
+        plan.wait(sp)
+        ts = []
+        for _ in range(5):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(3):
+                execute()
+            e1.record(stream)
+            plan.wait(sp)
+            ts.append(e0.elapsed_time(e1) / 3)
+        if first is None:
+            first = y.clone()
+        else:
+            err = (torch.linalg.vector_norm(y - first) / torch.linalg.vector_norm(first)).item()
+            if err > 3e-7 * 20:
+                raise SystemExit(f"ragged {env}: result differs from the first candidate ({err:.3g})")
+        params = f"batched_min={bmin};split={split};tile_run={run};order={order}"
+        results.append((16.0 * points / (min(ts) * 1e-3) / 1e9, params))
+        print(f"  ragged {params}: {min(ts):.3f} ms, {results[-1][0]:.0f} GB/s, {plan.tasks} persistent tasks",
+              flush=True)
+        plan.close()
+    best = max(results)
+    print(f"ragged best {best[1]} {best[0]:.0f} GB/s ({len(results)} candidates)", flush=True)
+    return f"20\tragged\t{best[1]}\t{best[0]:.0f}\t{len(items)}"
+
+
+def line_key(line: str):
+    """Lines of one size and batch (so one regime), or the ragged line, replace each other."""
+    f = line.split("\t")
+    return ("ragged",) if f[1] == "ragged" else (int(f[0]), f[4])
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--min", type=int, default=0)
     ap.add_argument("--max", type=int, default=30)
     ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--ragged", action="store_true", help="tune the ragged plans' choices only, under the table at --out")
+    ap.add_argument("--no-ragged", action="store_true", help="skip the ragged step after the sizes")
     args = ap.parse_args(argv)
-    os.environ["ONEFFT_TUNE_FILE"] = os.path.join(ROOT, "no-tuning-table-while-tuning")
-    for k in ("ONEFFT_BATCH_VARIANT", "ONEFFT_PASS_SPLIT", "ONEFFT_PASS_COLS", "ONEFFT_PASS_LAYOUT"):
+    os.environ["ONEFFT_TUNE_FILE"] = (os.path.abspath(args.out) if args.ragged
+                                      else os.path.join(ROOT, "no-tuning-table-while-tuning"))
+    for k in ("ONEFFT_BATCH_VARIANT", "ONEFFT_PASS_SPLIT", "ONEFFT_PASS_COLS", "ONEFFT_PASS_LAYOUT") + RAGGED_KEYS:
         os.environ.pop(k, None)
 
     import torch
@@ -200,9 +283,22 @@ def main(argv=None):
         f"# written by python -m onefft_amd.tune on {props.name} ({time.strftime('%Y-%m-%d')})",
     ]
 
+    kept = []
+    if os.path.exists(args.out):
+        kept = [l.rstrip("\n") for l in open(args.out) if l.strip() and not l.startswith("#")]
+
     def write(lines):
+        new = {line_key(l) for l in lines}
+        merged = [l for l in kept if line_key(l) not in new] + lines
+        merged.sort(key=lambda l: (1, 0, "") if line_key(l) == ("ragged",) else (0, int(l.split("\t")[0]),
+                                                                                  l.split("\t")[4].zfill(12)))
         with open(args.out, "w") as f:
-            f.write("\n".join(header + lines) + "\n")
+            f.write("\n".join(header + merged) + "\n")
+
+    if args.ragged:
+        write([tune_ragged(F, torch, stream)])
+        print(f"wrote {args.out}", flush=True)
+        return
 
     for L in range(args.min, args.max + 1):
         n = 1 << L
@@ -220,6 +316,10 @@ def main(argv=None):
             torch.cuda.empty_cache()
             write(lines)  # after every size, so an interrupted run keeps what it measured
     print(f"wrote {args.out}", flush=True)
+    if not args.no_ragged:
+        import subprocess
+
+        subprocess.run([sys.executable, "-m", "onefft_amd.tune", "--ragged", "--out", args.out], cwd=ROOT, check=True)
 
 
 if __name__ == "__main__":

@@ -36,12 +36,20 @@ def test_batch_graph_replays(L, batch, inverse):
     g.close()
 
 
-def test_ragged_graph_replays():
+@pytest.mark.parametrize("order", ["0", "1", "2"])
+def test_ragged_graph_replays(monkeypatch, order):
+    """A ragged execution with batched groups (2^16 and 2^17 items) and the persistent kernel, in
+    each order (groups first, persistent kernel first, side by side on the plan's second stream,
+    forked and joined by events inside the capture), replays bit for bit."""
     items = F.ragged_items(9, 400, lo=2, hi=17)
     extent = items[-1][0] + (1 << items[-1][1])
     x = torch.from_numpy(O.normal(9, extent)).cuda()
     eager, out = torch.zeros_like(x), torch.zeros_like(x)
+    monkeypatch.setenv("ONEFFT_RAGGED_BATCHED_MIN", "16")
+    monkeypatch.setenv("ONEFFT_RAGGED_ORDER", order)
     rp = F.RaggedPlan(items)
+    monkeypatch.delenv("ONEFFT_RAGGED_BATCHED_MIN")
+    monkeypatch.delenv("ONEFFT_RAGGED_ORDER")
     rp.execute(x.data_ptr(), eager.data_ptr(), stream())
     rp.wait(stream())
     g = F.Graph.ragged(rp, x.data_ptr(), out.data_ptr())
@@ -55,7 +63,7 @@ def test_ragged_graph_replays():
         rp.wait(stream())
     # the eager ragged result is itself right
     y, xs = eager.cpu().numpy(), x.cpu().numpy()
-    for off, l, inv in items[:50]:
+    for off, l, inv in items[:50] + [it for it in items if it[1] >= 16][:10]:
         seg = slice(2 * off, 2 * (off + (1 << l)))
         assert O.rel_rms(y[seg], O.fft_f64(xs[seg], 1 << l, bool(inv))) < 4e-7 + 1e-7 * l
 

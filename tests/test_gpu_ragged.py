@@ -92,6 +92,7 @@ def test_split_and_tile_run_bitwise(monkeypatch, split, tile_run):
     sizes = [20, 19, 18, 17, 14, 13, 10, 4] * 2
     items, extent = layout(sizes, gap=1, seed=9)
     x = O.normal(34, extent)
+    monkeypatch.setenv("ONEFFT_RAGGED_BATCHED_MIN", "21")  # every item in the persistent kernel
     want, _ = run(items, extent, x)
     monkeypatch.setenv("ONEFFT_RAGGED_SPLIT", split)
     monkeypatch.setenv("ONEFFT_RAGGED_TILE_RUN", tile_run)
@@ -99,6 +100,28 @@ def test_split_and_tile_run_bitwise(monkeypatch, split, tile_run):
     monkeypatch.delenv("ONEFFT_RAGGED_SPLIT")
     monkeypatch.delenv("ONEFFT_RAGGED_TILE_RUN")
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("batched_min,order", [("15", "0"), ("17", "1"), ("20", "2"), ("16", "2"), ("21", "2")])
+def test_batched_groups(monkeypatch, batched_min, order):
+    """Items of 2^batched_min points and more run as offset-batched multi-pass groups (one per size
+    and direction), the rest in the persistent kernel (21: none batched), in each order (groups
+    first, persistent kernel first, side by side on two streams): every item within the FP64 bar,
+    nothing written between items, and the persistent kernel's task count falls as more sizes are
+    batched."""
+    sizes = list(range(0, 21)) + list(range(20, 11, -1)) * 2 + [15, 16, 17] * 3
+    items, extent = layout(sizes, gap=5, seed=13)
+    x = O.normal(35, extent)
+    monkeypatch.setenv("ONEFFT_RAGGED_ORDER", order)
+    monkeypatch.setenv("ONEFFT_RAGGED_BATCHED_MIN", batched_min)
+    y, p = run(items, extent, x)
+    tasks = p.tasks
+    p.close()
+    check_items(items, x, y, extent)
+    monkeypatch.setenv("ONEFFT_RAGGED_BATCHED_MIN", "21")
+    q = F.RaggedPlan(items)
+    assert (tasks < q.tasks) == (batched_min != "21"), (batched_min, tasks, q.tasks)
+    q.close()
 
 
 def test_rejects_bad_items():
@@ -147,8 +170,10 @@ def test_phase_wait_timeout_is_reported(monkeypatch):
     d_in = torch.from_numpy(x).cuda()
     d_out = torch.zeros(2 * extent, device="cuda")
     monkeypatch.setenv("ONEFFT_RAGGED_SPIN_LIMIT", "0")
+    monkeypatch.setenv("ONEFFT_RAGGED_BATCHED_MIN", "21")  # the 2^20 item in the persistent kernel
     p = F.RaggedPlan(items)
     monkeypatch.delenv("ONEFFT_RAGGED_SPIN_LIMIT")
+    monkeypatch.delenv("ONEFFT_RAGGED_BATCHED_MIN")
     p.execute(d_in.data_ptr(), d_out.data_ptr(), stream())
     with pytest.raises(F.OneFFTError, match="timed out"):
         p.wait(stream())

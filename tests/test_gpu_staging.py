@@ -230,10 +230,18 @@ def test_allocation_failure_paths():
         assert rc in (0, -1)
         return rc == 0
 
-    items = (F.RaggedItem * 2)(F.RaggedItem(0, 10, 0), F.RaggedItem(1024, 16, 1))
+    # one item for the persistent kernel and two batched sizes, side by side: the plan's table, its
+    # second stream and two events, then a plan per batched size (a failure frees what came before)
+    items = (F.RaggedItem * 3)(F.RaggedItem(0, 10, 0), F.RaggedItem(1024, 16, 1), F.RaggedItem(1024 + 65536, 20, 0))
 
     def ragged():
-        p = L.onefft_ragged_plan_create(ctypes.addressof(items), 2)
+        os.environ["ONEFFT_RAGGED_BATCHED_MIN"] = "16"
+        os.environ["ONEFFT_RAGGED_ORDER"] = "2"
+        try:
+            p = L.onefft_ragged_plan_create(ctypes.addressof(items), 3)
+        finally:
+            os.environ.pop("ONEFFT_RAGGED_BATCHED_MIN", None)
+            os.environ.pop("ONEFFT_RAGGED_ORDER", None)
         if p:
             L.onefft_ragged_plan_destroy(p)
         return bool(p)
@@ -264,7 +272,7 @@ def test_allocation_failure_paths():
     # plan table + ONEFFT_STAGING_SLOTS slots (stream + 5 events each, then ring and buffers)
     assert counts["plan_create"] >= 10 and counts["forward"] == counts["plan_create"]
     assert counts["cache_prepare"] == counts["plan_create"]
-    assert counts["ragged_plan_create"] == 1 and counts["graph_create_batch"] == 1
+    assert counts["ragged_plan_create"] == 6 and counts["graph_create_batch"] == 1
     graph_plan.close()
 
 

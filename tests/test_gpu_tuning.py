@@ -72,6 +72,83 @@ def test_malformed_table_is_ignored(tmp_path):
                              "99\tsingle\tnt=1,layout=0\t1\t1\n")
 
 
+RAGGED_CHILD = textwrap.dedent("""
+    import json, os, sys
+    root, want, differs = sys.argv[1], json.loads(sys.argv[2]), sys.argv[3] == "1"
+    sys.path.insert(0, root)
+    import torch
+    import onefft_amd as F
+    from oracle import oracle as O
+    sizes = [20, 19, 18, 17, 16, 15, 14, 13, 12, 8, 4] * 2
+    items, end = [], 0
+    for i, l in enumerate(sizes):
+        items.append((end, l, i % 2))
+        end += (1 << l) + 3
+    def plan(env):
+        os.environ.update(env)
+        try:
+            return F.RaggedPlan(items)
+        finally:
+            for k in env:
+                os.environ.pop(k)
+    p = plan({})
+    ref = plan(want)
+    default = plan({"ONEFFT_RAGGED_BATCHED_MIN": "16", "ONEFFT_RAGGED_SPLIT": "19", "ONEFFT_RAGGED_TILE_RUN": "2"})
+    assert p.tasks == ref.tasks, (p.tasks, ref.tasks)
+    assert (p.tasks != default.tasks) == differs, (p.tasks, default.tasks)
+    x = O.normal(7, end)
+    d = torch.from_numpy(x).cuda()
+    o = torch.zeros_like(d)
+    s = torch.cuda.current_stream().cuda_stream
+    p.execute(d.data_ptr(), o.data_ptr(), s)
+    p.wait(s)
+    y = o.cpu().numpy()
+    for off, l, inv in items:
+        seg = slice(2 * off, 2 * (off + (1 << l)))
+        e = O.rel_rms(y[seg], O.fft_f64(x[seg], 1 << l, bool(inv)))
+        assert e < 4e-7 + 1e-7 * l, (l, inv, e)
+    print("ok")
+""")
+
+
+def run_ragged_with_table(tmp_path, text, want, differs):
+    """A ragged plan made under the table has the persistent-kernel task count of the choices
+    `want` (set through the environment, which overrides the table) and, if `differs`, not that of
+    the built-in defaults; its execution is within the FP64 bar."""
+    import json
+
+    table = tmp_path / "table.tsv"
+    table.write_text(text)
+    env = dict(os.environ, ONEFFT_TUNE_FILE=str(table))
+    for k in ("ONEFFT_RAGGED_BATCHED_MIN", "ONEFFT_RAGGED_SPLIT", "ONEFFT_RAGGED_TILE_RUN", "ONEFFT_RAGGED_ORDER"):
+        env.pop(k, None)
+    r = subprocess.run([sys.executable, "-c", RAGGED_CHILD, ROOT, json.dumps(want), "1" if differs else "0"],
+                       env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "ok" in r.stdout, r.stdout + r.stderr
+
+
+def test_table_ragged_choices_are_used_and_correct(tmp_path):
+    run_ragged_with_table(tmp_path, "20\tragged\tbatched_min=20;split=18;tile_run=1;order=2\t1\t1\n",
+                          {"ONEFFT_RAGGED_BATCHED_MIN": "20", "ONEFFT_RAGGED_SPLIT": "18", "ONEFFT_RAGGED_TILE_RUN": "1"},
+                          differs=True)
+
+
+def test_table_ragged_groups_off(tmp_path):
+    run_ragged_with_table(tmp_path, "20\tragged\tbatched_min=21;split=19;tile_run=2\t1\t1\n",
+                          {"ONEFFT_RAGGED_BATCHED_MIN": "21", "ONEFFT_RAGGED_SPLIT": "19", "ONEFFT_RAGGED_TILE_RUN": "2"},
+                          differs=True)
+
+
+def test_malformed_ragged_lines_are_ignored(tmp_path):
+    run_ragged_with_table(tmp_path, "20\tragged\tbatched_min=14;split=19;tile_run=2\t1\t1\n"  # batched single-pass sizes
+                                    "20\tragged\tbatched_min=17;split=19\t1\t1\n"              # a field missing
+                                    "20\tragged\tbatched_min=17;split=19;tile_run=0\t1\t1\n"   # no tiles per task
+                                    "20\tragged\tbatched_min=17;split=13;tile_run=2\t1\t1\n"  # split at a single-pass size
+                                    "20\tragged\tbatched_min=17;split=19;tile_run=2;order=3\t1\t1\n",  # no such order
+                          {"ONEFFT_RAGGED_BATCHED_MIN": "16", "ONEFFT_RAGGED_SPLIT": "19", "ONEFFT_RAGGED_TILE_RUN": "2"},
+                          differs=False)
+
+
 def test_committed_table_parses():
     path = os.path.join(ROOT, "onefft_amd", "tune_gfx950.tsv")
     if not os.path.exists(path):
@@ -80,7 +157,7 @@ def test_committed_table_parses():
         if line.startswith("#") or not line.strip():
             continue
         log2n, kind, params, gbs, batch = line.rstrip("\n").split("\t")
-        assert 0 <= int(log2n) <= 30 and kind in ("single", "multi") and float(gbs) > 0 and int(batch) >= 1
+        assert 0 <= int(log2n) <= 30 and kind in ("single", "multi", "ragged") and float(gbs) > 0 and int(batch) >= 1
 
 
 def test_committed_table_multi_choices():

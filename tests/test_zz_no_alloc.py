@@ -1,7 +1,13 @@
 """The allocation-free ragged entry point onefft_ragged_execute_items, through the C ABI, against
 onefft_ragged_execute (the planned entry point) on the same items: bit-identical outputs, two calls
 in a row with the second on a smaller input, every library allocation made to fail meanwhile
-(ONEFFT_FAIL_ALLOC=0) and no device memory taken; invalid items and too many items reported."""
+(ONEFFT_FAIL_ALLOC=0) and no device memory taken; invalid items and too many items reported.
+
+The allocation-free entry point runs every item in the persistent kernel (it sees the items only in
+device memory, so it cannot group the large ones on the host), so the planned execution it is
+compared with bit for bit is made with the batched groups off (ONEFFT_RAGGED_BATCHED_MIN=21); the
+default planned execution, whose large items run through the multi-pass kernels, is checked
+against it within the FP64 bar."""
 import ctypes
 import os
 
@@ -47,9 +53,15 @@ def every_size_items():
     return items
 
 
-def planned(items, x):
+def planned(items, x, groups=False):
+    """onefft_ragged_execute of a plan of `items`; groups=False: every item in the persistent kernel."""
     y = torch.zeros_like(x)
-    rp = F.RaggedPlan(items)
+    if not groups:
+        os.environ["ONEFFT_RAGGED_BATCHED_MIN"] = "21"
+    try:
+        rp = F.RaggedPlan(items)
+    finally:
+        os.environ.pop("ONEFFT_RAGGED_BATCHED_MIN", None)
     rp.execute(x.data_ptr(), y.data_ptr(), stream())
     rp.wait(stream())
     rp.close()
@@ -101,6 +113,12 @@ def test_no_alloc_matches_planned_execution():
         for o, l, _ in items:
             covered[o:o + (1 << l)] = True
         assert torch.equal(y[covered].view(torch.float32), want[covered].view(torch.float32)), "differs from onefft_ragged_execute"
+        # the default planned execution (large items through the multi-pass kernels) agrees
+        grouped = planned(items, x, groups=True)
+        for o, l, _ in items[:: max(1, len(items) // 200)]:
+            a, b = y[o:o + (1 << l)], grouped[o:o + (1 << l)]
+            d = float((a - b).abs().pow(2).sum().sqrt() / b.abs().pow(2).sum().sqrt().clamp_min(1e-30))
+            assert d < 2 * (4e-7 + 1e-7 * l), (o, l, d)
         assert bool((y[~covered] == complex(-1.0, -1.0)).all()), "wrote outside the items"
 
 

@@ -206,25 +206,34 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, const cf *__restric
     constexpr int S = stage_count(LOG2NF, LOG2P);
     constexpr uint32_t SPAN = pass_span(LOG2NF, C, LAYOUT);  // the columns start on distinct banks
 
+    // A first pass stores each column's outputs as one row, and the rows of neighbouring columns lie
+    // far apart, so a wave's direct stores write C pieces of 64 / C values each. With C >= 8 (pieces
+    // under 128 bytes) the final stage's outputs go through LDS instead, and each wave stores 64
+    // consecutive values of one row.
+    constexpr bool STAGE_OUT = FIRST && C >= 8 && S > 1;
     const uint32_t c = t % C;
     const uint32_t j = t / C;
     const uint32_t col = tile * C + c;
     // a column's region: SPAN complex values, or with the halves layout SPAN floats
     cf *L = lds + c * SPAN;
     float *Lf = reinterpret_cast<float *>(lds) + c * SPAN;
-
-    uint32_t in_base, in_step, out_base, out_step, low = 0;
-    if constexpr (FIRST) {
-        in_base = col;
-        in_step = a.columns;
-        uint32_t row = 0, rest = col;
+    // the output row of input column `cl` (first pass): its digits j_1.. (j_1 fastest) weighted
+    auto row_of = [&](uint32_t cl) {
+        uint32_t row = 0, rest = cl;
         static_for<0, kMaxPasses - 1>([&](auto d) {  // static indexes: the arguments stay in registers
             if (decltype(d)::value < a.ndig) {
                 row += (rest & ((1u << a.dig_log2[d]) - 1)) << a.dig_shift[d];
                 rest >>= a.dig_log2[d];
             }
         });
-        out_base = row << LOG2NF;
+        return row;
+    };
+
+    uint32_t in_base, in_step, out_base, out_step, low = 0;
+    if constexpr (FIRST) {
+        in_base = col;
+        in_step = a.columns;
+        out_base = row_of(col) << LOG2NF;
         out_step = 1;
     } else {
         low = col & ((1u << a.log2m) - 1);
@@ -295,7 +304,49 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, const cf *__restric
             dft<R>(&v[qq * R]);
         });
 
-        if constexpr (s == S - 1) {
+        if constexpr (s == S - 1 && STAGE_OUT) {
+            // output j + T q + STRIDE m of column c goes to its region's natural position; then
+            // value k of thread t is tile element g = k * THREADS + t: column g / NF, position
+            // g mod NF, stored to that column's row
+            constexpr uint32_t THREADS = C * T;
+            auto put = [&](auto part) {
+                static_for<0, BF>([&](auto q) {
+                    static_for<0, R>([&](auto m) {
+                        constexpr int qq = decltype(q)::value, mm = decltype(m)::value;
+                        const uint32_t at = lds_index(j + T * qq + mm * STRIDE);
+                        if constexpr (LAYOUT == kHalves)
+                            Lf[at] = decltype(part)::value ? v[qq * R + mm].i : v[qq * R + mm].r;
+                        else
+                            L[at] = v[qq * R + mm];
+                    });
+                });
+            };
+            __syncthreads();  // every read of the previous exchange is done
+            put(std::false_type{});
+            __syncthreads();
+            if constexpr (LAYOUT == kHalves) {
+                const float *lf0 = reinterpret_cast<const float *>(lds);
+                float re[P];
+                static_for<0, P>([&](auto k) {
+                    const uint32_t g = decltype(k)::value * THREADS + t;
+                    re[decltype(k)::value] = lf0[(g >> LOG2NF) * SPAN + lds_index(g & (NF - 1))];
+                });
+                __syncthreads();
+                put(std::true_type{});
+                __syncthreads();
+                static_for<0, P>([&](auto k) {
+                    const uint32_t g = decltype(k)::value * THREADS + t;
+                    const cf w = mk(re[decltype(k)::value], lf0[(g >> LOG2NF) * SPAN + lds_index(g & (NF - 1))]);
+                    dst[((uint64_t)row_of(tile * C + (g >> LOG2NF)) << LOG2NF) + (g & (NF - 1))] = maybe_swap<SWAP_OUT>(w);
+                });
+            } else {
+                static_for<0, P>([&](auto k) {
+                    const uint32_t g = decltype(k)::value * THREADS + t;
+                    const cf w = lds[(g >> LOG2NF) * SPAN + lds_index(g & (NF - 1))];
+                    dst[((uint64_t)row_of(tile * C + (g >> LOG2NF)) << LOG2NF) + (g & (NF - 1))] = maybe_swap<SWAP_OUT>(w);
+                });
+            }
+        } else if constexpr (s == S - 1) {
             static_for<0, BF>([&](auto q) {
                 static_for<0, R>([&](auto m) {
                     constexpr int qq = decltype(q)::value, mm = decltype(m)::value;

@@ -50,11 +50,15 @@ def parse():
                     help="instead of the benchmark: 3 executions of the 4 GiB ragged prefix (seed-3 items, "
                          "random input resident in HBM), for counter runs")
     ap.add_argument("--ragged-breakdown", action="store_true",
-                    help="instead of the benchmark: per size class of the 4 GiB ragged prefix, one ragged launch "
-                         "of that class's items against one standalone batched launch of as many transforms")
+                    help="instead of the benchmark: the 4 GiB ragged prefix's grouped execution and its parts, then "
+                         "per size class the persistent kernel over that class's items against one standalone "
+                         "batched launch of as many transforms")
     ap.add_argument("--report", action="store_true",
                     help="add a per-configuration report: errors against the reference CPU transform and the "
                          "FP64 oracle, and the reference's CPU times (several minutes of host work)")
+    ap.add_argument("--access-patterns", action="store_true",
+                    help="instead of the benchmark: plain copies with the global access patterns of the 2^27 "
+                         "passes (onefft_amd/libonefft_stream.so: onefft_access_pattern), 2 x 1 GiB")
     ap.add_argument("--variants", type=str, default=None,
                     help="instead of the benchmark: interleaved A/B of the single-pass kernel variants at these "
                          "log2 sizes (comma list), 512 MiB in and out each")
@@ -141,20 +145,54 @@ def batched_equivalent(torch, F, stream, items, x, y, reps=3):
     return out
 
 
+def tuned_ragged_batched_min() -> int:
+    """The smallest size (log2) that ragged plans run as batched groups: ONEFFT_RAGGED_BATCHED_MIN,
+    else the tuning table's ragged line, else the built-in default (kernels.hpp)."""
+    if os.environ.get("ONEFFT_RAGGED_BATCHED_MIN"):
+        return int(os.environ["ONEFFT_RAGGED_BATCHED_MIN"])
+    path = os.environ.get("ONEFFT_TUNE_FILE") or os.path.join(ROOT, "onefft_amd", "tune_gfx950.tsv")
+    try:
+        for line in open(path):
+            f = line.split("\t")
+            if len(f) >= 3 and f[1] == "ragged" and f[2].startswith("batched_min="):
+                return int(f[2].split(";")[0][len("batched_min="):])
+    except OSError:
+        pass
+    return 16
+
+
 def ragged_breakdown(torch, F):
-    """--ragged-breakdown: the 4 GiB ragged prefix, per size class."""
+    """--ragged-breakdown: the 4 GiB ragged prefix: the grouped execution and its two parts, then
+    per size class the persistent kernel over that class's items against one batched launch."""
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
     items = F.ragged_items(3, 200000, max_bytes=4 << 30)
     extent = items[-1][0] + (1 << items[-1][1])
     x = torch.randn(extent, dtype=torch.complex64, device="cuda")
     y = torch.empty_like(x)
-    rp = F.RaggedPlan(items)
-    one = lambda: rp.execute(x.data_ptr(), y.data_ptr(), sp)
-    one()
-    rp.wait(sp)
-    full = time_ms(torch, stream, one, 5)
-    rp.close()
+
+    def timed(its, env=None, reps=5):
+        os.environ.update(env or {})
+        try:
+            p = F.RaggedPlan(its)
+        finally:
+            for k in env or {}:
+                os.environ.pop(k, None)
+        run = lambda: p.execute(x.data_ptr(), y.data_ptr(), sp)
+        run()
+        p.wait(sp)
+        ms = time_ms(torch, stream, run, reps)
+        p.wait(sp)
+        p.close()
+        return ms
+
+    bmin = tuned_ragged_batched_min()
+    persistent = {"ONEFFT_RAGGED_BATCHED_MIN": "21"}
+    parts = {"batched_min": bmin, "grouped_execution_ms": round(timed(items), 3),
+             "persistent_part_ms": round(timed([it for it in items if it[1] < bmin], persistent), 3),
+             "groups_part_ms": round(timed([it for it in items if it[1] >= bmin]), 3) if bmin <= 20 else 0.0,
+             "persistent_only_ms": round(timed(items, persistent), 3)}
+    print(json.dumps(parts), flush=True)
     batched = batched_equivalent(torch, F, stream, items, x, y)
     rows = []
     for l, (count, bms) in batched.items():
@@ -163,17 +201,11 @@ def ragged_breakdown(torch, F):
             if ll == l:
                 sub.append((off, l, inv))
                 off += 1 << l
-        p = F.RaggedPlan(sub)
-        run = lambda: p.execute(x.data_ptr(), y.data_ptr(), sp)
-        run()
-        p.wait(sp)
-        rms = time_ms(torch, stream, run, 3)
-        p.close()
-        rows.append({"log2n": l, "items": count, "ragged_ms": round(rms, 4), "batched_ms": round(bms, 4),
-                     "gb_s_ragged": round(16.0 * count * (1 << l) / (rms * 1e-3) / 1e9, 1)})
+        rms = timed(sub, persistent, 3)
+        rows.append({"log2n": l, "items": count, "persistent_ms": round(rms, 4), "batched_ms": round(bms, 4),
+                     "gb_s_persistent": round(16.0 * count * (1 << l) / (rms * 1e-3) / 1e9, 1)})
         print(json.dumps(rows[-1]), flush=True)
-    print(json.dumps({"ragged_prefix_one_launch_ms": round(full, 3),
-                      "sum_of_class_launches_ms": round(sum(r["ragged_ms"] for r in rows), 3),
+    print(json.dumps({"sum_of_class_launches_ms": round(sum(r["persistent_ms"] for r in rows), 3),
                       "sum_of_batched_launches_ms": round(sum(r["batched_ms"] for r in rows), 3)}), flush=True)
 
 
@@ -214,7 +246,7 @@ def extra_workloads(torch, F, stream):
         inv = lambda: pl.batch_ptr(y.data_ptr(), z.data_ptr(), 1, n, n, True, sp)
         fwd(), inv()
         torch.cuda.synchronize()
-        reps = 20 if L < 27 else 5
+        reps = 20
         # as many untimed launches first: after host-only work the GPU's clocks ramp up again
         time_ms(torch, stream, fwd, reps)
         tf, ti = time_ms(torch, stream, fwd, reps), time_ms(torch, stream, inv, reps)
@@ -382,7 +414,7 @@ def report(torch, F, stream, x2, y2, ms2, threads):
         inv = lambda: pl.batch_ptr(yd.data_ptr(), zd.data_ptr(), 1, n, n, True, sp)
         fwd(), inv()
         torch.cuda.synchronize()
-        reps = 20 if L < 27 else 5
+        reps = 20
         # as many untimed launches first: after host-only work the GPU's clocks ramp up again
         time_ms(torch, stream, fwd, reps)
         tf, ti = time_ms(torch, stream, fwd, reps), time_ms(torch, stream, inv, reps)
@@ -522,6 +554,48 @@ def vendor_fft(torch, x, ours, stream, warmup=5, reps=50):
             "alg_gb_s": round(16 * n * batch / (us * 1e-6) / 1e9, 1), "rel_l2_vs_onefft": err}
 
 
+def access_patterns(torch):
+    """--access-patterns: the copy kernels of onefft_amd/libonefft_stream.so with the global access
+    patterns of the passes of a 2^27-point transform (split 2^8 * 2^9 * 2^10) and no arithmetic,
+    interleaved, 6 samples of 20 launches each; the two row-store patterns must write the same
+    permutation of the input."""
+    import ctypes
+    import statistics
+
+    lib = ctypes.CDLL(os.path.join(ROOT, "onefft_amd", "libonefft_stream.so"))
+    lib.onefft_access_pattern.restype = ctypes.c_int
+    lib.onefft_access_pattern.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    stream = torch.cuda.current_stream()
+    a = torch.randn(1 << 27, dtype=torch.complex64, device="cuda")
+    b = torch.zeros_like(a)
+    names = {0: "input-pass reads (16 columns x 1024 rows, 1 MiB apart), contiguous stores",
+             1: "input-pass reads, direct row stores (a wave writes 16 rows x 4 values)",
+             2: "input-pass reads, row stores through LDS (a wave writes 64 values of one row)",
+             3: "last in-place pass (32 columns x 256 rows, 4 MiB apart), in place"}
+
+    def run(p):
+        if lib.onefft_access_pattern(p, a.data_ptr(), b.data_ptr(), stream.cuda_stream) != 0:
+            raise RuntimeError("onefft_access_pattern failed")
+
+    run(1)
+    direct = b.clone()
+    run(2)
+    same = bool(torch.equal(direct, b))
+    del direct
+    for p in names:
+        for _ in range(5):
+            run(p)
+    times = {p: [] for p in names}
+    for _ in range(6):
+        for p in names:
+            times[p].append(time_ms(torch, stream, lambda: run(p), 20))
+    for p, ts in times.items():
+        print(json.dumps({"pattern": p, "what": names[p], "min_us": round(min(ts) * 1e3, 1),
+                          "median_us": round(statistics.median(ts) * 1e3, 1),
+                          "tb_s_at_min": round(2 * (8 << 27) / (min(ts) * 1e-3) / 1e12, 2),
+                          "row_stores_equal": same if p in (1, 2) else None}), flush=True)
+
+
 def stream_copy(torch, src, dst, stream, warmup=50, reps=200):
     """The copy kernel of onefft_amd/libonefft_stream.so: the 4096-point kernel's global access
     pattern (8-byte loads and stores, 32 KiB per workgroup, nontemporal) without its arithmetic,
@@ -633,6 +707,9 @@ def main():
 
     import onefft_amd as F
 
+    if args.access_patterns:
+        access_patterns(torch)
+        return
     if args.variants:
         variants_ab(torch, F, [int(v) for v in args.variants.split(",")])
         return

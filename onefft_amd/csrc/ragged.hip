@@ -29,6 +29,7 @@ struct RaggedGroup {
     bool inverse;
     uint32_t count;
     const uint64_t *offsets;   // device memory: the items' offsets, `count` entries
+    int lane;                  // split orders: 0 the caller's stream, 1 the plan's stream
 };
 
 struct onefft_ragged_plan {
@@ -39,7 +40,9 @@ struct onefft_ragged_plan {
     onefft::RaggedArgs args{};  // the persistent kernel's items (args.ntasks == 0: none)
     uint32_t grid = 0;
     uint32_t order = 0;              // onefft::RaggedOrder
-    hipStream_t side = nullptr;      // kConcurrent: the persistent kernel's stream
+    hipStream_t side = nullptr;      // kConcurrent: the persistent kernel's stream; split orders:
+                                     // lane 1's stream
+    int persistent_lane = 0;         // kSplitAll: the lane whose groups the persistent kernel follows
     hipEvent_t fork = nullptr, join = nullptr;
     std::vector<RaggedGroup> groups;          // largest size first
     std::vector<onefft_plan *> sized_plans;   // owned: one per group size
@@ -71,11 +74,12 @@ RaggedChoice ragged_choice() {
         const int v = std::atoi(env);
         if (v >= 1 && v <= 256) c.tile_run = (uint32_t)v;
     }
-    // ONEFFT_RAGGED_ORDER: 0 groups first, 1 persistent kernel first, 2 side by side
+    // ONEFFT_RAGGED_ORDER: 0 groups first, 1 persistent kernel first, 2 side by side, 3 and 4 the
+    // groups split by direction over two streams (runtime.hpp: RaggedOrder)
     c.order = t.set ? t.order : (uint32_t)kGroupsFirst;
     if (const char *env = std::getenv("ONEFFT_RAGGED_ORDER")) {
         const int v = std::atoi(env);
-        if (v >= 0 && v <= (int)kConcurrent) c.order = (uint32_t)v;
+        if (v >= 0 && v < (int)kRaggedOrders) c.order = (uint32_t)v;
     }
     // the bounded wait for phase 1: ~3 s of 0.2 us sleeps (ONEFFT_RAGGED_SPIN_LIMIT overrides it,
     // so tests can make the wait give up and check that the failure is reported)
@@ -281,8 +285,11 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
     a.spin_limit = choice.spin_limit;
     plan->grid = ragged_resident_grid();
     plan->order = choice.order;
-    // side by side needs both parts, a stream for the persistent kernel and two events
-    if (plan->order == kConcurrent && a.ntasks && !group_off.empty()) {
+    // side by side needs both parts, a stream for the persistent kernel and two events; the split
+    // orders need two groups or more
+    if (plan->order >= kSplitGroups && group_off.size() < 2) plan->order = kGroupsFirst;
+    if (plan->order == kConcurrent && !(a.ntasks && !group_off.empty())) plan->order = kGroupsFirst;
+    if (plan->order == kConcurrent || plan->order >= kSplitGroups) {
         if (stream_create(&plan->side) != hipSuccess || event_create(&plan->fork) != hipSuccess ||
             event_create(&plan->join) != hipSuccess) {
             set_error("ragged plan stream or event creation failed");
@@ -306,7 +313,17 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
             plan->sized_plans.push_back(sp);
         }
         plan->groups.push_back({sp, group_key[g].second, (uint32_t)group_off[g].size(),
-                                reinterpret_cast<const uint64_t *>(d + o_group[g])});
+                                reinterpret_cast<const uint64_t *>(d + o_group[g]), 0});
+    }
+    // split orders: each group (largest first) to the lane with fewer bytes so far, so that one
+    // lane's launches fill the other's drain between kernels
+    if (plan->order >= kSplitGroups) {
+        uint64_t bytes[2] = {0, 0};
+        for (RaggedGroup &g : plan->groups) {
+            g.lane = bytes[1] < bytes[0] ? 1 : 0;
+            bytes[g.lane] += (uint64_t)g.count * onefft_plan_size(g.plan);
+        }
+        plan->persistent_lane = bytes[1] < bytes[0] ? 1 : 0;
     }
     return plan;
 }
@@ -360,10 +377,11 @@ extern "C" int onefft_ragged_execute(const onefft_ragged_plan *plan, const float
     }
     const cf *in = reinterpret_cast<const cf *>(d_in);
     cf *out = reinterpret_cast<cf *>(d_out);
-    // the batched groups, largest first
-    auto groups = [&]() -> int {
+    // the batched groups, largest first: all (lane < 0), or those of one lane
+    auto groups = [&](hipStream_t s, int lane = -1) -> int {
         for (const RaggedGroup &g : plan->groups)
-            if (execute_device(g.plan, in, out, g.count, 0, 0, g.inverse, stream, g.offsets) != 0) return -1;
+            if (lane < 0 || g.lane == lane)
+                if (execute_device(g.plan, in, out, g.count, 0, 0, g.inverse, s, g.offsets) != 0) return -1;
         return 0;
     };
     // the persistent kernel over the other items
@@ -388,17 +406,24 @@ extern "C" int onefft_ragged_execute(const onefft_ragged_plan *plan, const float
             set_error("ragged fork", e);
             return -1;
         }
-        const int rc = groups() | persistent(plan->side);
+        int rc;
+        if (plan->order == kConcurrent) {
+            rc = groups(stream) | persistent(plan->side);
+        } else {
+            rc = groups(stream, 0) | groups(plan->side, 1);
+            if (plan->order == kSplitAll) rc |= persistent(plan->persistent_lane ? plan->side : stream);
+        }
         e = hipEventRecord(plan->join, plan->side);
         if (e == hipSuccess) e = hipStreamWaitEvent(stream, plan->join, 0);
         if (e != hipSuccess) {
             set_error("ragged join", e);
             return -1;
         }
+        if (plan->order == kSplitGroups) rc |= persistent(stream);
         return rc;
     }
-    if (plan->order == kPersistentFirst) return persistent(stream) != 0 ? -1 : groups();
-    return groups() != 0 ? -1 : persistent(stream);
+    if (plan->order == kPersistentFirst) return persistent(stream) != 0 ? -1 : groups(stream);
+    return groups(stream) != 0 ? -1 : persistent(stream);
 }
 
 extern "C" int onefft_ragged_execute_per_item(const onefft_ragged_plan *plan, const float *d_in, float *d_out,

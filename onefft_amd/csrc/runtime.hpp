@@ -86,7 +86,12 @@ enum RaggedOrder : uint32_t {
     kGroupsFirst = 0,      // the groups, then the persistent kernel, on the caller's stream
     kPersistentFirst = 1,  // the persistent kernel, then the groups
     kConcurrent = 2,       // the persistent kernel on the plan's own stream beside the groups
+    kSplitGroups = 3,      // the groups over two streams (the caller's and the plan's), balanced by
+                           // bytes, then the persistent kernel after both
+    kSplitAll = 4,         // as kSplitGroups, with the persistent kernel after the groups of the
+                           // stream that has fewer bytes
 };
+constexpr uint32_t kRaggedOrders = 5;
 struct RaggedChoice {
     uint32_t batched_min, split, tile_run, order, spin_limit;
 };

@@ -6,7 +6,9 @@
 // "split=a,b[,c[,d]];cols=x,y[,..][;layout=0|2]", log2 N_1 first; layout: the passes' LDS layout,
 // padded when absent) or "ragged" (params "batched_min=b;split=s;tile_run=t[;order=o]": ragged
 // plans' choices, one line, log2n the largest item size of the workload it was tuned on, batch its
-// item count; order 0 groups first, 1 persistent kernel first, 2 side by side). The batch the line was measured at sets its working-set regime
+// item count; order 0 groups first, 1 persistent kernel first, 2 side by side, 3 the groups split
+// over two streams, then the persistent kernel, 4 as 3 with the persistent kernel on the stream
+// with fewer bytes). The batch the line was measured at sets its working-set regime
 // (runtime.hpp: tune_regime); a size may have a line per regime. Written by `python -m onefft_amd.tune`; read once, at the first plan creation,
 // from $ONEFFT_TUNE_FILE or tune_gfx950.tsv next to the library. Plan creation stays O(n) with no
 // measuring, as with the reference's planner (its tuning is compiled in, src/onefft.c:652-697).
@@ -67,7 +69,7 @@ void parse_line(const char *line) {
         if (std::sscanf(params, "batched_min=%u;split=%u;tile_run=%u;order=%u", &b, &s, &t, &o) < 3) return;
         if (b <= (unsigned)kMaxSinglePassLog2 || b > (unsigned)kRaggedMaxLog2 + 1) return;
         if (s <= (unsigned)kRaggedSingleMaxLog2 || s > (unsigned)kRaggedMaxLog2 + 1) return;
-        if (t < 1 || t > 256 || o > kConcurrent) return;
+        if (t < 1 || t > 256 || o >= kRaggedOrders) return;
         g_ragged.set = true;
         g_ragged.batched_min = b;
         g_ragged.split = s;

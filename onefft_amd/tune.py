@@ -33,8 +33,9 @@ table just written: the batched groups use its multi-pass choices): the workload
 prefix of the seed-3 ragged stream (log2 n uniform on 4..20, directions 50/50); the candidates are
 the smallest size run as offset-batched multi-pass groups (2^15..2^20, or none), and, where the
 persistent kernel keeps items of 2^18 points or more, the split size and the tiles per pass task,
-and, where both parts exist, their order: groups first, persistent kernel first, or side by side on
-two streams (ONEFFT_RAGGED_BATCHED_MIN / _SPLIT / _TILE_RUN / _ORDER). Each execution is timed like a size's candidates
+and, where both parts exist, their order: groups first, persistent kernel first, side by side on
+two streams, or the groups split over two streams, balanced by bytes, with the persistent kernel
+after both or after the lighter one (ONEFFT_RAGGED_BATCHED_MIN / _SPLIT / _TILE_RUN / _ORDER). Each execution is timed like a size's candidates
 and checked against the first candidate's output; the fastest is written as the table's ragged line.
 """
 from __future__ import annotations
@@ -182,7 +183,7 @@ def ragged_candidates():
     for bmin in (15, 16, 17, 18, 19, 20, 21):
         shapes = [(19, 2)] + [(split, run) for split in (18, 19, 20) if split < bmin for run in (1, 2, 4)]
         for split, run in dict.fromkeys(shapes):
-            for order in ((0, 1, 2) if bmin <= 20 else (0,)):
+            for order in ((0, 1, 2, 3, 4) if bmin <= 20 else (0,)):
                 out.append((bmin, split, run, order))
     return out
 

@@ -36,11 +36,12 @@ def test_batch_graph_replays(L, batch, inverse):
     g.close()
 
 
-@pytest.mark.parametrize("order", ["0", "1", "2"])
+@pytest.mark.parametrize("order", ["0", "1", "2", "3", "4"])
 def test_ragged_graph_replays(monkeypatch, order):
     """A ragged execution with batched groups (2^16 and 2^17 items) and the persistent kernel, in
     each order (groups first, persistent kernel first, side by side on the plan's second stream,
-    forked and joined by events inside the capture), replays bit for bit."""
+    the groups split over both streams; forked and joined by events inside the capture), replays
+    bit for bit."""
     items = F.ragged_items(9, 400, lo=2, hi=17)
     extent = items[-1][0] + (1 << items[-1][1])
     x = torch.from_numpy(O.normal(9, extent)).cuda()

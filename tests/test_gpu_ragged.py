@@ -102,11 +102,13 @@ def test_split_and_tile_run_bitwise(monkeypatch, split, tile_run):
     assert np.array_equal(got, want)
 
 
-@pytest.mark.parametrize("batched_min,order", [("15", "0"), ("17", "1"), ("20", "2"), ("16", "2"), ("21", "2")])
+@pytest.mark.parametrize("batched_min,order", [("15", "0"), ("17", "1"), ("20", "2"), ("16", "2"), ("21", "2"),
+                                                ("15", "3"), ("20", "3"), ("16", "4"), ("20", "4"), ("21", "3")])
 def test_batched_groups(monkeypatch, batched_min, order):
     """Items of 2^batched_min points and more run as offset-batched multi-pass groups (one per size
     and direction), the rest in the persistent kernel (21: none batched), in each order (groups
-    first, persistent kernel first, side by side on two streams): every item within the FP64 bar,
+    first, persistent kernel first, side by side on two streams, the groups split over two streams
+    with the persistent kernel after both or after one of them): every item within the FP64 bar,
     nothing written between items, and the persistent kernel's task count falls as more sizes are
     batched."""
     sizes = list(range(0, 21)) + list(range(20, 11, -1)) * 2 + [15, 16, 17] * 3

@@ -144,7 +144,7 @@ def test_malformed_ragged_lines_are_ignored(tmp_path):
                                     "20\tragged\tbatched_min=17;split=19\t1\t1\n"              # a field missing
                                     "20\tragged\tbatched_min=17;split=19;tile_run=0\t1\t1\n"   # no tiles per task
                                     "20\tragged\tbatched_min=17;split=13;tile_run=2\t1\t1\n"  # split at a single-pass size
-                                    "20\tragged\tbatched_min=17;split=19;tile_run=2;order=3\t1\t1\n",  # no such order
+                                    "20\tragged\tbatched_min=17;split=19;tile_run=2;order=5\t1\t1\n",  # no such order
                           {"ONEFFT_RAGGED_BATCHED_MIN": "16", "ONEFFT_RAGGED_SPLIT": "19", "ONEFFT_RAGGED_TILE_RUN": "2"},
                           differs=False)
 

@@ -145,9 +145,10 @@ size_t onefft_ragged_extent(const onefft_ragged_plan *plan);
 size_t onefft_ragged_task_count(const onefft_ragged_plan *plan);
 
 /* Transforms every item, asynchronously on `stream`: the grouped items by multi-pass launches at
- * their offsets, the rest by one persistent-kernel launch, either after them on `stream` or beside
- * them on the plan's own stream (forked from and joined back to `stream` by events, so work queued
- * on `stream` afterwards, a graph capture of the call included, sees every item done). d_in and
+ * their offsets, the rest by one persistent-kernel launch, in the tuned order: on `stream` alone,
+ * or with the plan's own second stream taking the persistent kernel or part of the groups (forked
+ * from and joined back to `stream` by events, so work queued on `stream` afterwards, a graph
+ * capture of the call included, sees every item done). d_in and
  * d_out are device memory, 8-byte aligned, each holding onefft_ragged_extent() complex values,
  * not overlapping (-1 if they do); d_in is not modified. A ragged plan runs one execution at a
  * time: executions of one plan must be ordered (same stream or synchronised). Returns 0 or -1. */

@@ -44,13 +44,14 @@ template <int LOG2N>
 static hipError_t launch_batch_n(bool inverse, const BatchArgs &a, BatchVariant v, hipStream_t stream) {
     if constexpr (LOG2N <= 5) {
         // contiguous batches of up to 32 points: coalesced blocks (k_batch.hpp: fft_small_kernel)
-        if (a.batch > 0 && a.in_dist == (1u << LOG2N) && a.out_dist == (1u << LOG2N) && a.batch <= (~0ull >> 5))
+        if (!a.done && a.batch > 0 && a.in_dist == (1u << LOG2N) && a.out_dist == (1u << LOG2N) &&
+            a.batch <= (~0ull >> 5))
             return launch_small<LOG2N>(inverse, v, a, stream);
     }
     constexpr int PER = batch_per_wg(LOG2N);
     const uint64_t blocks = (a.batch + PER - 1) / PER;
     if (blocks == 0) return hipSuccess;
-    if (blocks > 0x7fffffffu) return hipErrorInvalidValue;
+    if (blocks > 0x7fffffffu || (a.done && blocks != 1)) return hipErrorInvalidValue;
     const dim3 grid((uint32_t)blocks);
     constexpr bool USES_LDS = stage_count(LOG2N, batch_log2p(LOG2N)) > 1;
     // FP64-base levels (fft_common.hpp: stage_uses_tw64), padded layout only; a level above the

@@ -11,7 +11,8 @@ template <int LOG2N, bool INV, bool NT, int LAYOUT, int TW64>
 __global__ __launch_bounds__(batch_wg(LOG2N)) void fft_batch_kernel(const cf *__restrict__ in, cf *__restrict__ out,
                                                                       const cf *__restrict__ tw,
                                                                       const cd *__restrict__ tw64, uint64_t in_dist,
-                                                                      uint64_t out_dist, uint64_t batch) {
+                                                                      uint64_t out_dist, uint64_t batch,
+                                                                      uint32_t *done, uint32_t seq) {
     constexpr int N = 1 << LOG2N;
     constexpr uint32_t T = N >> batch_log2p(LOG2N);  // threads per transform
     constexpr int PER = batch_per_wg(LOG2N);         // transforms per workgroup
@@ -36,16 +37,22 @@ __global__ __launch_bounds__(batch_wg(LOG2N)) void fft_batch_kernel(const cf *__
         const PtrIO<NT> io{in + src * in_dist, out + which * out_dist};
         single_body<LOG2N, INV, LAYOUT, TW64>(io, which < batch, tw, lds + (USES_LDS ? c * SPAN : 0), t % T, tw64);
     }
+    if (done) {
+        // every thread's stores reach the system before the barrier, the flag after it
+        __threadfence_system();
+        __syncthreads();
+        if (t == 0) __hip_atomic_store(done, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
 template <int LOG2N, bool NT, int LAYOUT, int TW64 = 0>
 inline void launch_variant(bool inverse, const BatchArgs &a, dim3 grid, hipStream_t stream) {
     if (inverse)
         hipLaunchKernelGGL((fft_batch_kernel<LOG2N, true, NT, LAYOUT, TW64>), grid, dim3(batch_wg(LOG2N)), 0, stream,
-                           a.in, a.out, a.tw, a.tw64, a.in_dist, a.out_dist, a.batch);
+                           a.in, a.out, a.tw, a.tw64, a.in_dist, a.out_dist, a.batch, a.done, a.seq);
     else
         hipLaunchKernelGGL((fft_batch_kernel<LOG2N, false, NT, LAYOUT, TW64>), grid, dim3(batch_wg(LOG2N)), 0, stream,
-                           a.in, a.out, a.tw, a.tw64, a.in_dist, a.out_dist, a.batch);
+                           a.in, a.out, a.tw, a.tw64, a.in_dist, a.out_dist, a.batch, a.done, a.seq);
 }
 
 // Contiguous batches of transforms of n <= 32 points (in_dist = out_dist = n; the reference's small

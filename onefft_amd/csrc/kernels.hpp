@@ -32,6 +32,11 @@ struct BatchArgs {
     uint64_t in_dist;    // complex values between transforms
     uint64_t out_dist;
     uint64_t batch;
+    // if set (launches of one workgroup only): the workgroup's last act stores `seq` to *done,
+    // after its outputs are visible system-wide, so a host thread polling that word in coherent
+    // host memory sees the transform complete without waiting for the launch's completion signal
+    uint32_t *done;
+    uint32_t seq;
 };
 
 // Kernel variant of the single-pass transform: nontemporal global access, the LDS layout

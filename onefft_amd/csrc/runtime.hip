@@ -198,15 +198,20 @@ DeviceGuard::~DeviceGuard() {
 
 // Launches the transform of `batch` arrays on `stream`; data pointers are device memory.
 int execute_device(const onefft_plan *plan, const cf *d_in, cf *d_out, uint64_t batch, uint64_t in_dist,
-                   uint64_t out_dist, bool inverse, hipStream_t stream, const uint64_t *d_offsets) {
+                   uint64_t out_dist, bool inverse, hipStream_t stream, const uint64_t *d_offsets, uint32_t *done,
+                   uint32_t seq) {
     if (batch == 0) return 0;
     const int regime = tune_regime(plan->n, batch);
     if (d_offsets && !plan->multi) {
         set_error("transforms at listed offsets need a multi-pass plan");
         return -1;
     }
+    if (done && (plan->multi || batch != 1)) {
+        set_error("a completion word needs one single-pass transform");
+        return -1;
+    }
     if (!plan->multi) {
-        BatchArgs a{d_in, d_out, plan->tw_single, plan->tw64_single, in_dist, out_dist, batch};
+        BatchArgs a{d_in, d_out, plan->tw_single, plan->tw64_single, in_dist, out_dist, batch, done, seq};
         hipError_t e = launch_batch((int)plan->log2n, inverse, a, plan->variant[regime], stream);
         if (e != hipSuccess) {
             set_error("single-pass launch", e);

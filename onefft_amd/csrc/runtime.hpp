@@ -23,8 +23,11 @@ void set_error(const char *what);
 // Launch `batch` transforms on device memory, asynchronously on `stream`. With d_offsets (device
 // memory, `batch` entries; multi-pass plans only), transform b is at d_offsets[b] complex values
 // in both buffers and the distances are ignored.
+// With `done` (device address of a word in coherent host memory; one single-pass transform only),
+// the kernel stores `seq` there once its output is visible to the host (kernels.hpp: BatchArgs).
 int execute_device(const onefft_plan *plan, const cf *d_in, cf *d_out, uint64_t batch, uint64_t in_dist,
-                   uint64_t out_dist, bool inverse, hipStream_t stream, const uint64_t *d_offsets = nullptr);
+                   uint64_t out_dist, bool inverse, hipStream_t stream, const uint64_t *d_offsets = nullptr,
+                   uint32_t *done = nullptr, uint32_t seq = 0);
 
 // onefft_plan_create without its side effect on the staging pool (staging = false): the plans
 // that ragged plans hold for their batched groups. Freed by onefft_plan_destroy.

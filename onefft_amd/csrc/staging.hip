@@ -35,6 +35,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -56,6 +57,10 @@ constexpr size_t kRingMinBytes = kChunk;                          // smaller: HI
 constexpr int kMaxSlots = 64;
 constexpr int kMaxDevices = 16;
 constexpr int kDefaultSlots = 4;
+constexpr size_t kWordBytes = 256;  // the direct buffer's completion word, on its own cache line
+// how long a direct call polls its completion word before it waits on the stream instead (the
+// GPU may be busy with other work); the result is the same either way
+constexpr std::chrono::microseconds kPollFor{200};
 
 // what a call needs besides the stream
 enum Need : unsigned { kNeedBuf = 1, kNeedRing = 2, kNeedDirect = 4 };
@@ -73,8 +78,10 @@ struct Slot {
     void *buf = nullptr;  // 2 * cap bytes: input then output
     size_t cap = 0;
     char *pinned = nullptr;      // kRing * kChunk bytes of pinned host memory
-    char *direct = nullptr;      // 2 * kDirectBytes of coherent pinned host memory: input, output
+    char *direct = nullptr;      // 2 * kDirectBytes of coherent pinned host memory: input, output,
+                                 // then the completion word
     char *direct_dev = nullptr;  // the same memory as the device addresses it
+    uint32_t seq = 0;            // the last value the kernel was asked to store in the word
 
     // the stream and events; all or nothing
     bool create() {
@@ -109,7 +116,7 @@ struct Slot {
         }
         if ((need & kNeedDirect) && !direct) {
             void *p = nullptr, *d = nullptr;
-            if (host_malloc(&p, 2 * kDirectBytes, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+            if (host_malloc(&p, 2 * kDirectBytes + kWordBytes, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
                 set_error("pinned staging allocation failed");
                 return false;
             }
@@ -121,6 +128,8 @@ struct Slot {
             }
             direct = static_cast<char *>(p);
             direct_dev = static_cast<char *>(d);
+            __atomic_store_n(reinterpret_cast<uint32_t *>(direct + 2 * kDirectBytes), 0u, __ATOMIC_RELEASE);
+            seq = 0;
         }
         if ((need & kNeedBuf) && cap < bytes) {
             void *p = nullptr;
@@ -332,6 +341,18 @@ bool stage_out(Slot &s, void *dst, const void *d_src, size_t bytes) {
     return true;
 }
 
+// Polls the completion word until it holds `seq` (true) or kPollFor has passed (false).
+bool poll_word(const uint32_t *word, uint32_t seq) {
+    const auto until = std::chrono::steady_clock::now() + kPollFor;
+    for (;;) {
+        for (int i = 0; i < 64; i++) {
+            if (__atomic_load_n(word, __ATOMIC_ACQUIRE) == seq) return true;
+            __builtin_ia32_pause();
+        }
+        if (std::chrono::steady_clock::now() > until) return false;
+    }
+}
+
 int transform(Slot &s, const onefft_plan *plan, const float *in, float *out, bool inverse) {
     const size_t n = onefft_plan_size(plan);
     const size_t bytes = n * 2 * sizeof(float);
@@ -357,17 +378,29 @@ int transform(Slot &s, const onefft_plan *plan, const float *in, float *out, boo
         }
     }
     if (small_pageable) {
+        // the kernel reads the input from and writes the output to coherent host memory, then
+        // stores a new sequence number to the completion word; polling that word returns ~4 us
+        // sooner than waiting for the launch's completion signal
         std::memcpy(s.direct, in, bytes);
         const cf *h_in = reinterpret_cast<const cf *>(s.direct_dev);
         cf *h_out = reinterpret_cast<cf *>(s.direct_dev + kDirectBytes);
-        if (execute_device(plan, h_in, h_out, 1, n, n, inverse, s.stream) != 0) {
+        uint32_t *word = reinterpret_cast<uint32_t *>(s.direct + 2 * kDirectBytes);
+        uint32_t *word_dev = reinterpret_cast<uint32_t *>(s.direct_dev + 2 * kDirectBytes);
+        const uint32_t seq = ++s.seq == 0 ? ++s.seq : s.seq;  // never 0, the word's initial value
+        if (execute_device(plan, h_in, h_out, 1, n, n, inverse, s.stream, nullptr, word_dev, seq) != 0) {
             (void)hipStreamSynchronize(s.stream);
             return -1;
         }
-        const hipError_t e = hipStreamSynchronize(s.stream);
-        if (e != hipSuccess) {
-            set_error("transform", e);
-            return -1;
+        if (!poll_word(word, seq)) {
+            const hipError_t e = hipStreamSynchronize(s.stream);
+            if (e != hipSuccess) {
+                set_error("transform", e);
+                return -1;
+            }
+            if (__atomic_load_n(word, __ATOMIC_ACQUIRE) != seq) {
+                set_error("transform completed without its completion word");
+                return -1;
+            }
         }
         std::memcpy(out, s.direct + kDirectBytes, bytes);
         return 0;

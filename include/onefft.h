@@ -16,9 +16,9 @@
  * On this implementation the transform runs on the GPU. The drop-in functions accept host
  * pointers (copied to and from the device around the transform, which then dominates the cost)
  * and device pointers (hipMalloc'd memory, transformed in place on the device); they return when
- * `out` holds the result. A call with a device-memory argument is ordered like a synchronous
- * hipMemcpy: after all work queued before it on the null (default) stream and on blocking
- * streams; work on other non-blocking streams must be synchronised by the caller first. The
+ * `out` holds the result. A call with a device-memory argument first waits, like a synchronous
+ * hipMemcpy, for all work queued before it on the null (default) stream; work on other streams
+ * must be synchronised by the caller first. The
  * second part of the header adds asynchronous device-pointer, batched and ragged entry points on
  * a caller-supplied HIP stream.
  *

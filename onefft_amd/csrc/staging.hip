@@ -16,10 +16,10 @@
 // Per argument:
 //   * device memory, 8-byte aligned: transformed where it lies;
 //   * device memory, misaligned: copied device to device into the staging buffer and back;
-//   * pinned host memory: copied by DMA straight to and from the staging buffers;
-//   * ordinary (pageable) host memory, transforms of one LDS pass (n <= 2^14, 128 KiB): copied by
+//   * host memory, pageable or pinned, transforms of one LDS pass (n <= 2^14, 128 KiB): copied by
 //     the CPU into the slot's coherent pinned buffer, which the kernel reads and writes directly
 //     over the host link (no DMA transfers, whose set-up latency dominates at these sizes);
+//   * pinned host memory, larger: copied by DMA straight to and from the staging buffers;
 //   * ordinary (pageable) host memory up to 4 MiB: HIP's own pageable copies (lower set-up latency
 //     than the ring below at these sizes);
 //   * ordinary (pageable) host memory, larger: a pipeline through the pinned ring. The input is cut into
@@ -28,9 +28,9 @@
 //     and the link run at the same time; the CPU copies are split over a small thread pool
 //     (CopyPool). (HIP's own pageable copies run at ~30 GB/s host to device and ~18 GB/s back on
 //     MI355X hosts; pinned DMA at ~57 GB/s each way; one copying thread ~28 GB/s.)
-// A call with a device-memory argument first orders its stream after all work queued so far on
-// the null stream (and, through it, on every blocking stream), like a synchronous hipMemcpy, so a
-// producer kernel on the caller's default stream finishes before the transform reads.
+// A call with a device-memory argument first waits for all work queued so far on the null stream,
+// like a synchronous hipMemcpy, so a producer kernel on the caller's default stream finishes
+// before the transform reads.
 // The output is written only after the transform has completed without error.
 #include <hip/hip_runtime.h>
 
@@ -73,7 +73,6 @@ unsigned needs_for_size(size_t bytes) {
 struct Slot {
     std::atomic<bool> busy{false};
     hipStream_t stream = nullptr;
-    hipEvent_t order = nullptr;  // recorded on the null stream before a device-memory call
     hipEvent_t ev[kRing] = {};
     void *buf = nullptr;  // 2 * cap bytes: input then output
     size_t cap = 0;
@@ -89,7 +88,7 @@ struct Slot {
             set_error("staging stream creation failed");
             return false;
         }
-        bool ok = event_create(&order) == hipSuccess;
+        bool ok = true;
         for (int i = 0; ok && i < kRing; i++) ok = event_create(&ev[i]) == hipSuccess;
         if (!ok) {
             set_error("staging event creation failed");
@@ -152,12 +151,10 @@ struct Slot {
         if (buf) (void)hipFree(buf);
         if (pinned) (void)hipHostFree(pinned);
         if (direct) (void)hipHostFree(direct);
-        if (order) (void)hipEventDestroy(order);
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
-        order = nullptr;
         for (auto &e : ev) e = nullptr;
         buf = nullptr;
         cap = 0;
@@ -361,23 +358,26 @@ int transform(Slot &s, const onefft_plan *plan, const float *in, float *out, boo
     // (which the API does not allow) is still copied first, so the result stays correct
     const bool in_direct = in_kind == Mem::kDevice && ((uintptr_t)in & 7) == 0 && disjoint(in, bytes, out, bytes);
     const bool out_direct = out_kind == Mem::kDevice && ((uintptr_t)out & 7) == 0;
-    const bool small_pageable = in_kind == Mem::kPageable && out_kind == Mem::kPageable && bytes <= kDirectBytes;
+    // host memory of one LDS-pass transform, pageable or pinned: through the coherent buffer (a CPU
+    // copy of up to 128 KiB costs less than setting up two DMA transfers)
+    const bool small_host = in_kind != Mem::kDevice && out_kind != Mem::kDevice && bytes <= kDirectBytes;
     unsigned need = 0;
-    if (small_pageable) need = kNeedDirect;
+    if (small_host) need = kNeedDirect;
     else if (!(in_direct && out_direct))
         need = kNeedBuf | ((in_kind == Mem::kPageable || out_kind == Mem::kPageable) && bytes >= kRingMinBytes ? kNeedRing : 0u);
     if (!s.serves(bytes, need) && !s.ensure(bytes, need)) return -1;
 
     if (in_kind == Mem::kDevice || out_kind == Mem::kDevice) {
-        // order after the work already queued on the null stream (a synchronous copy's rule)
-        hipError_t e = hipEventRecord(s.order, nullptr);
-        if (e == hipSuccess) e = hipStreamWaitEvent(s.stream, s.order, 0);
+        // order after the work already queued on the null stream (a synchronous copy's rule): the
+        // host waits for it, which costs nothing when it is idle (recording an event on the null
+        // stream for this stream to wait on cost ~16 us per call)
+        const hipError_t e = hipStreamSynchronize(nullptr);
         if (e != hipSuccess) {
             set_error("ordering after the null stream", e);
             return -1;
         }
     }
-    if (small_pageable) {
+    if (small_host) {
         // the kernel reads the input from and writes the output to coherent host memory, then
         // stores a new sequence number to the completion word; polling that word returns ~4 us
         // sooner than waiting for the launch's completion signal

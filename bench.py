@@ -231,8 +231,22 @@ def extra_workloads(torch, F, stream):
     xd = torch.from_numpy(xh).cuda()
     yd = torch.empty_like(xd)
     dev_ms = time_ms(torch, stream, lambda: p.batch_ptr(xd.data_ptr(), yd.data_ptr(), 1, n, n, False, sp), 2000)
+    # one synchronous forward call per buffer kind (host pageable, host pinned, device)
+    sync_us = {}
+    xp, yp = torch.from_numpy(xh).pin_memory(), torch.from_numpy(yh).pin_memory()
+    torch.cuda.synchronize()
+    for kind, (a, b) in (("pageable", (xh.ctypes.data, yh.ctypes.data)), ("pinned", (xp.data_ptr(), yp.data_ptr())),
+                         ("device", (xd.data_ptr(), yd.data_ptr()))):
+        for _ in range(20):
+            p.forward_ptr(a, b)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            p.forward_ptr(a, b)
+        sync_us[kind] = round((time.perf_counter() - t0) / reps * 1e6, 2)
     out["n1024"] = {"host_fwd_inv_us": round(host_us, 2), "device_fwd_us": round(dev_ms * 1e3, 2),
-                    "note": "host: synchronous onefft_plan_forward+inverse on host buffers (PCIe copies included)"}
+                    "sync_fwd_us": sync_us,
+                    "note": "host: synchronous onefft_plan_forward+inverse on host buffers (PCIe copies included); "
+                            "sync_fwd_us: one synchronous onefft_plan_forward per buffer kind"}
 
     # large single transforms, forward then inverse
     large = []

@@ -350,6 +350,31 @@ bool poll_word(const uint32_t *word, uint32_t seq) {
     }
 }
 
+// One single-pass transform on s.stream whose kernel then stores a new sequence number to the
+// slot's completion word; returns once the word holds it, or, past kPollFor, once the stream has
+// finished (polling returns ~4 us sooner than waiting for the launch's completion signal).
+bool run_with_word(Slot &s, const onefft_plan *plan, const cf *d_in, cf *d_out, bool inverse) {
+    const size_t n = onefft_plan_size(plan);
+    uint32_t *word = reinterpret_cast<uint32_t *>(s.direct + 2 * kDirectBytes);
+    uint32_t *word_dev = reinterpret_cast<uint32_t *>(s.direct_dev + 2 * kDirectBytes);
+    const uint32_t seq = ++s.seq == 0 ? ++s.seq : s.seq;  // never 0, the word's initial value
+    if (execute_device(plan, d_in, d_out, 1, n, n, inverse, s.stream, nullptr, word_dev, seq) != 0) {
+        (void)hipStreamSynchronize(s.stream);
+        return false;
+    }
+    if (poll_word(word, seq)) return true;
+    const hipError_t e = hipStreamSynchronize(s.stream);
+    if (e != hipSuccess) {
+        set_error("transform", e);
+        return false;
+    }
+    if (__atomic_load_n(word, __ATOMIC_ACQUIRE) != seq) {
+        set_error("transform completed without its completion word");
+        return false;
+    }
+    return true;
+}
+
 int transform(Slot &s, const onefft_plan *plan, const float *in, float *out, bool inverse) {
     const size_t n = onefft_plan_size(plan);
     const size_t bytes = n * 2 * sizeof(float);
@@ -358,11 +383,14 @@ int transform(Slot &s, const onefft_plan *plan, const float *in, float *out, boo
     // (which the API does not allow) is still copied first, so the result stays correct
     const bool in_direct = in_kind == Mem::kDevice && ((uintptr_t)in & 7) == 0 && disjoint(in, bytes, out, bytes);
     const bool out_direct = out_kind == Mem::kDevice && ((uintptr_t)out & 7) == 0;
-    // host memory of one LDS-pass transform, pageable or pinned: through the coherent buffer (a CPU
-    // copy of up to 128 KiB costs less than setting up two DMA transfers)
-    const bool small_host = in_kind != Mem::kDevice && out_kind != Mem::kDevice && bytes <= kDirectBytes;
+    // transforms of one LDS pass: the kernel stores the completion word in the coherent buffer;
+    // host memory, pageable or pinned, goes through that buffer too (a CPU copy of up to 128 KiB
+    // costs less than setting up two DMA transfers)
+    const bool small = bytes <= kDirectBytes;
+    const bool small_host = small && in_kind != Mem::kDevice && out_kind != Mem::kDevice;
+    const bool small_device = small && in_direct && out_direct;
     unsigned need = 0;
-    if (small_host) need = kNeedDirect;
+    if (small_host || small_device) need = kNeedDirect;
     else if (!(in_direct && out_direct))
         need = kNeedBuf | ((in_kind == Mem::kPageable || out_kind == Mem::kPageable) && bytes >= kRingMinBytes ? kNeedRing : 0u);
     if (!s.serves(bytes, need) && !s.ensure(bytes, need)) return -1;
@@ -378,33 +406,16 @@ int transform(Slot &s, const onefft_plan *plan, const float *in, float *out, boo
         }
     }
     if (small_host) {
-        // the kernel reads the input from and writes the output to coherent host memory, then
-        // stores a new sequence number to the completion word; polling that word returns ~4 us
-        // sooner than waiting for the launch's completion signal
+        // the kernel reads the input from and writes the output to coherent host memory
         std::memcpy(s.direct, in, bytes);
         const cf *h_in = reinterpret_cast<const cf *>(s.direct_dev);
         cf *h_out = reinterpret_cast<cf *>(s.direct_dev + kDirectBytes);
-        uint32_t *word = reinterpret_cast<uint32_t *>(s.direct + 2 * kDirectBytes);
-        uint32_t *word_dev = reinterpret_cast<uint32_t *>(s.direct_dev + 2 * kDirectBytes);
-        const uint32_t seq = ++s.seq == 0 ? ++s.seq : s.seq;  // never 0, the word's initial value
-        if (execute_device(plan, h_in, h_out, 1, n, n, inverse, s.stream, nullptr, word_dev, seq) != 0) {
-            (void)hipStreamSynchronize(s.stream);
-            return -1;
-        }
-        if (!poll_word(word, seq)) {
-            const hipError_t e = hipStreamSynchronize(s.stream);
-            if (e != hipSuccess) {
-                set_error("transform", e);
-                return -1;
-            }
-            if (__atomic_load_n(word, __ATOMIC_ACQUIRE) != seq) {
-                set_error("transform completed without its completion word");
-                return -1;
-            }
-        }
+        if (!run_with_word(s, plan, h_in, h_out, inverse)) return -1;
         std::memcpy(out, s.direct + kDirectBytes, bytes);
         return 0;
     }
+    if (small_device)
+        return run_with_word(s, plan, reinterpret_cast<const cf *>(in), reinterpret_cast<cf *>(out), inverse) ? 0 : -1;
     const cf *d_in = reinterpret_cast<const cf *>(in);
     cf *d_out = reinterpret_cast<cf *>(out);
     if (!in_direct) {

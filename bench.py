@@ -188,10 +188,13 @@ def ragged_breakdown(torch, F):
 
     bmin = tuned_ragged_batched_min()
     persistent = {"ONEFFT_RAGGED_BATCHED_MIN": "21"}
+    # every item in the persistent kernel, with as many workgroups as are resident (the tuned grid
+    # is the grouped execution's, for its small items)
+    persistent_all = {"ONEFFT_RAGGED_BATCHED_MIN": "21", "ONEFFT_RAGGED_GRID": "0"}
     parts = {"batched_min": bmin, "grouped_execution_ms": round(timed(items), 3),
              "persistent_part_ms": round(timed([it for it in items if it[1] < bmin], persistent), 3),
              "groups_part_ms": round(timed([it for it in items if it[1] >= bmin]), 3) if bmin <= 20 else 0.0,
-             "persistent_only_ms": round(timed(items, persistent), 3)}
+             "persistent_only_ms": round(timed(items, persistent_all), 3)}
     print(json.dumps(parts), flush=True)
     batched = batched_equivalent(torch, F, stream, items, x, y)
     rows = []
@@ -201,7 +204,7 @@ def ragged_breakdown(torch, F):
             if ll == l:
                 sub.append((off, l, inv))
                 off += 1 << l
-        rms = timed(sub, persistent, 3)
+        rms = timed(sub, persistent_all, 3)
         rows.append({"log2n": l, "items": count, "persistent_ms": round(rms, 4), "batched_ms": round(bms, 4),
                      "gb_s_persistent": round(16.0 * count * (1 << l) / (rms * 1e-3) / 1e9, 1)})
         print(json.dumps(rows[-1]), flush=True)
@@ -302,12 +305,15 @@ def extra_workloads(torch, F, stream):
     torch.cuda.synchronize()
     t_per = time_ms(torch, stream, per, 2)
     t_batched = sum(ms for _, ms in batched_equivalent(torch, F, stream, items, x, y).values())
-    # the same items with every item in the one persistent launch (no batched groups)
+    # the same items with every item in the one persistent launch (no batched groups), with as many
+    # workgroups as are resident (the tuned grid is the grouped execution's, for its small items)
     os.environ["ONEFFT_RAGGED_BATCHED_MIN"] = "21"
+    os.environ["ONEFFT_RAGGED_GRID"] = "0"
     try:
         rq = F.RaggedPlan(items)
     finally:
         os.environ.pop("ONEFFT_RAGGED_BATCHED_MIN", None)
+        os.environ.pop("ONEFFT_RAGGED_GRID", None)
     one_p = lambda: rq.execute(x.data_ptr(), y.data_ptr(), sp)
     one_p()
     rq.wait(sp)

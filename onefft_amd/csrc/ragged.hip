@@ -81,6 +81,13 @@ RaggedChoice ragged_choice() {
         const int v = std::atoi(env);
         if (v >= 0 && v < (int)kRaggedOrders) c.order = (uint32_t)v;
     }
+    // the persistent kernel's workgroups: the table's choice (0: as many as are resident at once);
+    // ONEFFT_RAGGED_GRID overrides it
+    c.grid = t.set ? t.grid : 0;
+    if (const char *env = std::getenv("ONEFFT_RAGGED_GRID")) {
+        const long g = std::atol(env);
+        if (g >= 0 && g <= (1L << 20)) c.grid = (uint32_t)g;
+    }
     // the bounded wait for phase 1: ~3 s of 0.2 us sleeps (ONEFFT_RAGGED_SPIN_LIMIT overrides it,
     // so tests can make the wait give up and check that the failure is reported)
     c.spin_limit = 1u << 24;
@@ -283,7 +290,11 @@ extern "C" onefft_ragged_plan *onefft_ragged_plan_create(const onefft_ragged_ite
     a.nphase1 = (uint32_t)nphase1;
     a.nitems = (uint32_t)nper;
     a.spin_limit = choice.spin_limit;
+    // the persistent kernel waits only for tasks that running workgroups have claimed, so any
+    // grid up to the resident count works; fewer workgroups publish and wait at the phase change
+    // fewer times (each publication writes back its XCD's L2)
     plan->grid = ragged_resident_grid();
+    if (choice.grid) plan->grid = std::min(plan->grid, choice.grid);
     plan->order = choice.order;
     // side by side needs both parts, a stream for the persistent kernel and two events; the split
     // orders need two groups or more
@@ -409,6 +420,8 @@ extern "C" int onefft_ragged_execute(const onefft_ragged_plan *plan, const float
         int rc;
         if (plan->order == kConcurrent) {
             rc = groups(stream) | persistent(plan->side);
+        } else if (plan->order == kSplitLead) {
+            rc = persistent(plan->side) | groups(stream, 0) | groups(plan->side, 1);
         } else {
             rc = groups(stream, 0) | groups(plan->side, 1);
             if (plan->order == kSplitAll) rc |= persistent(plan->persistent_lane ? plan->side : stream);

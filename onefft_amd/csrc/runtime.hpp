@@ -82,8 +82,8 @@ struct DeviceGuard {
 // The ragged executions' choices: the smallest size run as offset-batched multi-pass groups
 // (log2; kRaggedMaxLog2 + 1 for none), the split size (log2; two-pass items of the persistent
 // kernel from it up are spread over tile runs, smaller ones run whole), tiles per pass task, the
-// order of the groups and the persistent kernel (RaggedOrder), and the bounded wait's sleep
-// count. The allocation-free entry point uses split, tile run and wait only (its persistent
+// order of the groups and the persistent kernel (RaggedOrder), the persistent kernel's
+// workgroups, and the bounded wait's sleep count. The allocation-free entry point uses split, tile run and wait only (its persistent
 // kernel runs every item).
 enum RaggedOrder : uint32_t {
     kGroupsFirst = 0,      // the groups, then the persistent kernel, on the caller's stream
@@ -93,17 +93,20 @@ enum RaggedOrder : uint32_t {
                            // bytes, then the persistent kernel after both
     kSplitAll = 4,         // as kSplitGroups, with the persistent kernel after the groups of the
                            // stream that has fewer bytes
+    kSplitLead = 5,        // as kSplitGroups, with the persistent kernel before the groups of the
+                           // plan's stream
 };
-constexpr uint32_t kRaggedOrders = 5;
+constexpr uint32_t kRaggedOrders = 6;
 struct RaggedChoice {
     uint32_t batched_min, split, tile_run, order, spin_limit;
+    uint32_t grid;  // persistent workgroups (0: as many as are resident at once)
 };
 RaggedChoice ragged_choice();
 
 // The tuning table's ragged line (tuning.hip), if it has one.
 struct RaggedTune {
     bool set = false;
-    uint32_t batched_min = 0, split = 0, tile_run = 0, order = 0;
+    uint32_t batched_min = 0, split = 0, tile_run = 0, order = 0, grid = 0;
 };
 const RaggedTune &ragged_tune();
 

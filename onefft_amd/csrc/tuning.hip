@@ -4,11 +4,12 @@
 // File: tab-separated lines "log2n  kind  params  gb_s  batch", '#' comments, where kind is
 // "single" (params "nt=0|1,layout=0..2[,tw64=0..3[,xlane=0|1]]"), "multi" (params
 // "split=a,b[,c[,d]];cols=x,y[,..][;layout=0|2]", log2 N_1 first; layout: the passes' LDS layout,
-// padded when absent) or "ragged" (params "batched_min=b;split=s;tile_run=t[;order=o]": ragged
+// padded when absent) or "ragged" (params "batched_min=b;split=s;tile_run=t[;order=o][;grid=g]": ragged
 // plans' choices, one line, log2n the largest item size of the workload it was tuned on, batch its
 // item count; order 0 groups first, 1 persistent kernel first, 2 side by side, 3 the groups split
 // over two streams, then the persistent kernel, 4 as 3 with the persistent kernel on the stream
-// with fewer bytes). The batch the line was measured at sets its working-set regime
+// with fewer bytes, 5 as 3 with the persistent kernel first on the plan's stream; grid the
+// persistent kernel's workgroups, 0 or absent for as many as are resident at once). The batch the line was measured at sets its working-set regime
 // (runtime.hpp: tune_regime); a size may have a line per regime. Written by `python -m onefft_amd.tune`; read once, at the first plan creation,
 // from $ONEFFT_TUNE_FILE or tune_gfx950.tsv next to the library. Plan creation stays O(n) with no
 // measuring, as with the reference's planner (its tuning is compiled in, src/onefft.c:652-697).
@@ -70,11 +71,15 @@ void parse_line(const char *line) {
         if (b <= (unsigned)kMaxSinglePassLog2 || b > (unsigned)kRaggedMaxLog2 + 1) return;
         if (s <= (unsigned)kRaggedSingleMaxLog2 || s > (unsigned)kRaggedMaxLog2 + 1) return;
         if (t < 1 || t > 256 || o >= kRaggedOrders) return;
+        long g = 0;
+        if (const char *gp = std::strstr(params, "grid=")) g = std::atol(gp + 5);
+        if (g < 0 || g > (1L << 20)) return;
         g_ragged.set = true;
         g_ragged.batched_min = b;
         g_ragged.split = s;
         g_ragged.tile_run = t;
         g_ragged.order = o;
+        g_ragged.grid = (uint32_t)g;
         return;
     }
     const int regime = tune_regime(uint64_t(1) << log2n, batch);

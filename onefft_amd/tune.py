@@ -35,7 +35,8 @@ the smallest size run as offset-batched multi-pass groups (2^15..2^20, or none),
 persistent kernel keeps items of 2^18 points or more, the split size and the tiles per pass task,
 and, where both parts exist, their order: groups first, persistent kernel first, side by side on
 two streams, or the groups split over two streams, balanced by bytes, with the persistent kernel
-after both or after the lighter one (ONEFFT_RAGGED_BATCHED_MIN / _SPLIT / _TILE_RUN / _ORDER). Each execution is timed like a size's candidates
+after both, after the lighter one or first on the second stream; then, for the three best, fewer
+persistent workgroups (ONEFFT_RAGGED_BATCHED_MIN / _SPLIT / _TILE_RUN / _ORDER / _GRID). Each execution is timed like a size's candidates
 and checked against the first candidate's output; the fastest is written as the table's ragged line.
 """
 from __future__ import annotations
@@ -171,20 +172,23 @@ def tune_size(F, torch, timed, L: int, batch: int, single: bool) -> str:
     return f"{L}\t{best[1]}\t{best[2]}\t{best[0]:.0f}\t{batch}"
 
 
-RAGGED_KEYS = ("ONEFFT_RAGGED_BATCHED_MIN", "ONEFFT_RAGGED_SPLIT", "ONEFFT_RAGGED_TILE_RUN", "ONEFFT_RAGGED_ORDER")
+RAGGED_KEYS = ("ONEFFT_RAGGED_BATCHED_MIN", "ONEFFT_RAGGED_SPLIT", "ONEFFT_RAGGED_TILE_RUN", "ONEFFT_RAGGED_ORDER",
+               "ONEFFT_RAGGED_GRID")
+RAGGED_GRIDS = (384, 256, 192, 128)  # persistent workgroups tried for the best first-stage choices
 
 
 def ragged_candidates():
-    """(batched_min, split, tile_run, order) choices that give distinct executions: the split size
-    and the tile run matter only where the persistent kernel keeps items of the split size (split <
-    batched_min), the order only where there are groups (batched_min <= 20); elsewhere they are
-    left at the defaults."""
+    """(batched_min, split, tile_run, order, grid) choices that give distinct executions: the split
+    size and the tile run matter only where the persistent kernel keeps items of the split size
+    (split < batched_min), the order only where there are groups (batched_min <= 20); elsewhere they
+    are left at the defaults. The grid (persistent workgroups) is 0 here, as many as are resident;
+    smaller grids are tried in a second stage."""
     out = []
     for bmin in (15, 16, 17, 18, 19, 20, 21):
         shapes = [(19, 2)] + [(split, run) for split in (18, 19, 20) if split < bmin for run in (1, 2, 4)]
         for split, run in dict.fromkeys(shapes):
-            for order in ((0, 1, 2, 3, 4) if bmin <= 20 else (0,)):
-                out.append((bmin, split, run, order))
+            for order in ((0, 1, 2, 3, 4, 5) if bmin <= 20 else (0,)):
+                out.append((bmin, split, run, order, 0))
     return out
 
 
@@ -197,8 +201,10 @@ def tune_ragged(F, torch, stream) -> str:
     x = torch.randn(extent, dtype=torch.complex64, device="cuda")
     y = torch.empty_like(x)
     first, results = None, []
-    for bmin, split, run, order in ragged_candidates():
-        env = dict(zip(RAGGED_KEYS, (str(bmin), str(split), str(run), str(order))))
+
+    def measure(bmin, split, run, order, grid):
+        nonlocal first
+        env = dict(zip(RAGGED_KEYS, (str(bmin), str(split), str(run), str(order), str(grid))))
         os.environ.update(env)
         try:
             plan = F.RaggedPlan(items)
@@ -224,11 +230,19 @@ def tune_ragged(F, torch, stream) -> str:
             err = (torch.linalg.vector_norm(y - first) / torch.linalg.vector_norm(first)).item()
             if err > 3e-7 * 20:
                 raise SystemExit(f"ragged {env}: result differs from the first candidate ({err:.3g})")
-        params = f"batched_min={bmin};split={split};tile_run={run};order={order}"
-        results.append((16.0 * points / (min(ts) * 1e-3) / 1e9, params))
+        params = f"batched_min={bmin};split={split};tile_run={run};order={order}" + (f";grid={grid}" if grid else "")
+        results.append((16.0 * points / (min(ts) * 1e-3) / 1e9, params, (bmin, split, run, order)))
         print(f"  ragged {params}: {min(ts):.3f} ms, {results[-1][0]:.0f} GB/s, {plan.tasks} persistent tasks",
               flush=True)
         plan.close()
+
+    for cand in ragged_candidates():
+        measure(*cand)
+    # second stage: fewer persistent workgroups for the three best choices (each publishes and
+    # waits once at the phase change)
+    for _, _, choice in sorted(results, reverse=True)[:3]:
+        for grid in RAGGED_GRIDS:
+            measure(*choice, grid)
     best = max(results)
     print(f"ragged best {best[1]} {best[0]:.0f} GB/s ({len(results)} candidates)", flush=True)
     return f"20\tragged\t{best[1]}\t{best[0]:.0f}\t{len(items)}"

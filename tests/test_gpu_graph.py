@@ -36,7 +36,7 @@ def test_batch_graph_replays(L, batch, inverse):
     g.close()
 
 
-@pytest.mark.parametrize("order", ["0", "1", "2", "3", "4"])
+@pytest.mark.parametrize("order", ["0", "1", "2", "3", "4", "5"])
 def test_ragged_graph_replays(monkeypatch, order):
     """A ragged execution with batched groups (2^16 and 2^17 items) and the persistent kernel, in
     each order (groups first, persistent kernel first, side by side on the plan's second stream,

@@ -85,6 +85,25 @@ def test_two_pass_items_many_tiles():
     check_items(items, x, y, extent)
 
 
+@pytest.mark.parametrize("grid,order", [("1", "0"), ("96", "3"), ("192", "4"), ("256", "5")])
+def test_persistent_grid_bitwise(monkeypatch, grid, order):
+    """Fewer persistent workgroups (ONEFFT_RAGGED_GRID; one workgroup included, which then runs
+    every task of both phases itself) change only which workgroup computes what: with every item in
+    the persistent kernel, and with groups in the split orders, the results equal the default
+    grid's bit for bit."""
+    sizes = [20, 19, 18, 17, 16, 15, 14, 13, 10, 4] * 2
+    items, extent = layout(sizes, gap=1, seed=11)
+    x = O.normal(36, extent)
+    monkeypatch.setenv("ONEFFT_RAGGED_BATCHED_MIN", "21" if order == "0" else "16")
+    monkeypatch.setenv("ONEFFT_RAGGED_ORDER", order)
+    want, _ = run(items, extent, x)
+    monkeypatch.setenv("ONEFFT_RAGGED_GRID", grid)
+    got, _ = run(items, extent, x)
+    monkeypatch.delenv("ONEFFT_RAGGED_GRID")
+    assert np.array_equal(got, want)
+    check_items(items, x, got, extent)
+
+
 @pytest.mark.parametrize("split,tile_run", [("20", "1"), ("19", "3"), ("18", "256"), ("21", "2")])
 def test_split_and_tile_run_bitwise(monkeypatch, split, tile_run):
     """Other split sizes and tile-run lengths (ONEFFT_RAGGED_SPLIT / ONEFFT_RAGGED_TILE_RUN) change
@@ -103,7 +122,8 @@ def test_split_and_tile_run_bitwise(monkeypatch, split, tile_run):
 
 
 @pytest.mark.parametrize("batched_min,order", [("15", "0"), ("17", "1"), ("20", "2"), ("16", "2"), ("21", "2"),
-                                                ("15", "3"), ("20", "3"), ("16", "4"), ("20", "4"), ("21", "3")])
+                                                ("15", "3"), ("20", "3"), ("16", "4"), ("20", "4"), ("21", "3"),
+                                                ("16", "5"), ("19", "5")])
 def test_batched_groups(monkeypatch, batched_min, order):
     """Items of 2^batched_min points and more run as offset-batched multi-pass groups (one per size
     and direction), the rest in the persistent kernel (21: none batched), in each order (groups

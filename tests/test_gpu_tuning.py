@@ -120,7 +120,8 @@ def run_ragged_with_table(tmp_path, text, want, differs):
     table = tmp_path / "table.tsv"
     table.write_text(text)
     env = dict(os.environ, ONEFFT_TUNE_FILE=str(table))
-    for k in ("ONEFFT_RAGGED_BATCHED_MIN", "ONEFFT_RAGGED_SPLIT", "ONEFFT_RAGGED_TILE_RUN", "ONEFFT_RAGGED_ORDER"):
+    for k in ("ONEFFT_RAGGED_BATCHED_MIN", "ONEFFT_RAGGED_SPLIT", "ONEFFT_RAGGED_TILE_RUN", "ONEFFT_RAGGED_ORDER",
+              "ONEFFT_RAGGED_GRID"):
         env.pop(k, None)
     r = subprocess.run([sys.executable, "-c", RAGGED_CHILD, ROOT, json.dumps(want), "1" if differs else "0"],
                        env=env, capture_output=True, text=True, timeout=300)
@@ -130,6 +131,12 @@ def run_ragged_with_table(tmp_path, text, want, differs):
 def test_table_ragged_choices_are_used_and_correct(tmp_path):
     run_ragged_with_table(tmp_path, "20\tragged\tbatched_min=20;split=18;tile_run=1;order=2\t1\t1\n",
                           {"ONEFFT_RAGGED_BATCHED_MIN": "20", "ONEFFT_RAGGED_SPLIT": "18", "ONEFFT_RAGGED_TILE_RUN": "1"},
+                          differs=True)
+
+
+def test_table_ragged_grid_and_split_order(tmp_path):
+    run_ragged_with_table(tmp_path, "20\tragged\tbatched_min=17;split=19;tile_run=2;order=3;grid=128\t1\t1\n",
+                          {"ONEFFT_RAGGED_BATCHED_MIN": "17", "ONEFFT_RAGGED_SPLIT": "19", "ONEFFT_RAGGED_TILE_RUN": "2"},
                           differs=True)
 
 
@@ -144,7 +151,8 @@ def test_malformed_ragged_lines_are_ignored(tmp_path):
                                     "20\tragged\tbatched_min=17;split=19\t1\t1\n"              # a field missing
                                     "20\tragged\tbatched_min=17;split=19;tile_run=0\t1\t1\n"   # no tiles per task
                                     "20\tragged\tbatched_min=17;split=13;tile_run=2\t1\t1\n"  # split at a single-pass size
-                                    "20\tragged\tbatched_min=17;split=19;tile_run=2;order=5\t1\t1\n",  # no such order
+                                    "20\tragged\tbatched_min=17;split=19;tile_run=2;order=6\t1\t1\n"  # no such order
+                                    "20\tragged\tbatched_min=17;split=19;tile_run=2;order=3;grid=-4\t1\t1\n",  # bad grid
                           {"ONEFFT_RAGGED_BATCHED_MIN": "16", "ONEFFT_RAGGED_SPLIT": "19", "ONEFFT_RAGGED_TILE_RUN": "2"},
                           differs=False)
 

@@ -74,8 +74,8 @@ RaggedChoice ragged_choice() {
         const int v = std::atoi(env);
         if (v >= 1 && v <= 256) c.tile_run = (uint32_t)v;
     }
-    // ONEFFT_RAGGED_ORDER: 0 groups first, 1 persistent kernel first, 2 side by side, 3 and 4 the
-    // groups split by direction over two streams (runtime.hpp: RaggedOrder)
+    // ONEFFT_RAGGED_ORDER: 0 groups first, 1 persistent kernel first, 2 side by side, 3..5 the
+    // groups split over two streams (runtime.hpp: RaggedOrder)
     c.order = t.set ? t.order : (uint32_t)kGroupsFirst;
     if (const char *env = std::getenv("ONEFFT_RAGGED_ORDER")) {
         const int v = std::atoi(env);

@@ -338,6 +338,9 @@ def extra_workloads(torch, F, stream):
     g.close()
     rp.close()
     del x, y
+    torch.cuda.empty_cache()
+    # every item of the stream, in 4 GiB chunks (the prefix above is the first of them)
+    out["ragged"]["all_items"] = ragged_all_items(torch, F, stream)
 
     # spectral filter step: forward, filter, inverse, and the backward (two more transforms)
     B, n = 256, 65536
@@ -361,6 +364,44 @@ def extra_workloads(torch, F, stream):
                      "fft_alg_gb_s": round(16 * n * B / (t_fft * 1e-3) / 1e9, 1),
                      "note": "step = 4 transforms (2 forward, 2 backward) plus torch's elementwise ops"}
     return out
+
+
+def ragged_all_items(torch, F, stream):
+    """Every one of the 200000 seed-3 ragged items (184.6 GiB of input), in chunks of at most 4 GiB
+    of input packed back to back, one execution per chunk (after one untimed execution), each on
+    fresh N(0,1) input: the summed execution times."""
+    sp = stream.cuda_stream
+    allitems = F.ragged_items(3, 200000)
+    chunks, start = [], 0
+    while start < len(allitems):
+        base, end = allitems[start][0], start
+        while end < len(allitems) and (allitems[end][0] - base + (1 << allitems[end][1])) * 8 <= (4 << 30):
+            end += 1
+        chunks.append([(o - base, l, v) for o, l, v in allitems[start:end]])
+        start = end
+    extents = [ch[-1][0] + (1 << ch[-1][1]) for ch in chunks]
+    # buffers for the longest chunk: every chunk's items lie inside them
+    xd = torch.empty(max(extents), dtype=torch.complex64, device="cuda")
+    yd = torch.empty_like(xd)
+    total_ms, total_flops, total_bytes = 0.0, 0.0, 0.0
+    for ch, ext in zip(chunks, extents):
+        assert ext <= xd.numel() and ext <= yd.numel()
+        xd[:ext].normal_()
+        rp = F.RaggedPlan(ch)
+        assert rp.extent == ext
+        rp.execute(xd.data_ptr(), yd.data_ptr(), sp)
+        rp.wait(sp)
+        total_ms += time_ms(torch, stream, lambda: rp.execute(xd.data_ptr(), yd.data_ptr(), sp), 1)
+        rp.wait(sp)
+        rp.close()
+        total_flops += sum(5.0 * (1 << l) * l for _, l, _ in ch)
+        total_bytes += 16.0 * ext
+    del xd, yd
+    torch.cuda.empty_cache()
+    return {"items": len(allitems), "chunks": len(chunks), "gpu_ms": round(total_ms, 2),
+            "gflops": round(total_flops / (total_ms * 1e-3) / 1e9, 1),
+            "alg_gb_s": round(total_bytes / (total_ms * 1e-3) / 1e9, 1),
+            "gib_in": round(total_bytes / 2 / 2**30, 1)}
 
 
 def report(torch, F, stream, x2, y2, ms2, threads):
@@ -495,37 +536,7 @@ def report(torch, F, stream, x2, y2, ms2, threads):
     # 4b. every one of the 200000 items, in chunks of <= 4 GiB input, one launch per chunk
     del xd, yd
     torch.cuda.empty_cache()
-    allitems = F.ragged_items(3, 200000)
-    chunks, start = [], 0
-    while start < len(allitems):
-        base, end = allitems[start][0], start
-        while end < len(allitems) and (allitems[end][0] - base + (1 << allitems[end][1])) * 8 <= (4 << 30):
-            end += 1
-        chunks.append([(o - base, l, v) for o, l, v in allitems[start:end]])
-        start = end
-    extents = [ch[-1][0] + (1 << ch[-1][1]) for ch in chunks]
-    # buffers for the longest chunk: every chunk's items lie inside them
-    xd = torch.empty(max(extents), dtype=torch.complex64, device="cuda")
-    yd = torch.empty_like(xd)
-    total_ms, total_flops, total_bytes = 0.0, 0.0, 0.0
-    for ch, ext in zip(chunks, extents):
-        assert ext <= xd.numel() and ext <= yd.numel()
-        xd[:ext].normal_()
-        rp = F.RaggedPlan(ch)
-        assert rp.extent == ext
-        rp.execute(xd.data_ptr(), yd.data_ptr(), sp)
-        rp.wait(sp)
-        total_ms += time_ms(torch, stream, lambda: rp.execute(xd.data_ptr(), yd.data_ptr(), sp), 1)
-        rp.wait(sp)
-        rp.close()
-        total_flops += sum(5.0 * (1 << l) * l for _, l, _ in ch)
-        total_bytes += 16.0 * ext
-    out["4b_ragged_all_items"] = {"items": len(allitems), "chunks": len(chunks), "gpu_ms": round(total_ms, 2),
-                                  "gflops": round(total_flops / (total_ms * 1e-3) / 1e9, 1),
-                                  "alg_gb_s": round(total_bytes / (total_ms * 1e-3) / 1e9, 1),
-                                  "gib_in": round(total_bytes / 2 / 2**30, 1)}
-    del xd, yd
-    torch.cuda.empty_cache()
+    out["4b_ragged_all_items"] = ragged_all_items(torch, F, stream)
 
     progress("config 5")
     # 5. spectral filter: the FFT part, 256 x 65536 forward and inverse, N(0,1) seed 4

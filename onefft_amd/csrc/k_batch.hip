@@ -16,11 +16,11 @@ namespace onefft {
 template <int LOG2N, bool XLANE>
 static void launch_small_x(bool inverse, bool nt, const BatchArgs &a, dim3 grid, uint64_t total, hipStream_t stream) {
     if (inverse) {
-        if (nt) hipLaunchKernelGGL((fft_small_kernel<LOG2N, true, true, XLANE>), grid, dim3(256), 0, stream, a.in, a.out, total);
-        else hipLaunchKernelGGL((fft_small_kernel<LOG2N, true, false, XLANE>), grid, dim3(256), 0, stream, a.in, a.out, total);
+        if (nt) hipLaunchKernelGGL((fft_small_kernel<LOG2N, true, true, XLANE>), grid, dim3(256), 0, stream, a.in, a.out, total, a.done, a.seq);
+        else hipLaunchKernelGGL((fft_small_kernel<LOG2N, true, false, XLANE>), grid, dim3(256), 0, stream, a.in, a.out, total, a.done, a.seq);
     } else {
-        if (nt) hipLaunchKernelGGL((fft_small_kernel<LOG2N, false, true, XLANE>), grid, dim3(256), 0, stream, a.in, a.out, total);
-        else hipLaunchKernelGGL((fft_small_kernel<LOG2N, false, false, XLANE>), grid, dim3(256), 0, stream, a.in, a.out, total);
+        if (nt) hipLaunchKernelGGL((fft_small_kernel<LOG2N, false, true, XLANE>), grid, dim3(256), 0, stream, a.in, a.out, total, a.done, a.seq);
+        else hipLaunchKernelGGL((fft_small_kernel<LOG2N, false, false, XLANE>), grid, dim3(256), 0, stream, a.in, a.out, total, a.done, a.seq);
     }
 }
 
@@ -28,7 +28,7 @@ template <int LOG2N>
 static hipError_t launch_small(bool inverse, BatchVariant v, const BatchArgs &a, hipStream_t stream) {
     const uint64_t total = a.batch << LOG2N;
     const uint64_t blocks = (total + kSmallBlock - 1) / kSmallBlock;
-    if (blocks > 0x7fffffffu) return hipErrorInvalidValue;
+    if (blocks > 0x7fffffffu || (a.done && blocks != 1)) return hipErrorInvalidValue;
     const dim3 grid((uint32_t)blocks);
     if constexpr (LOG2N == 5) {
         if (v.xlane) {  // the DPP pair exchange exists for 32 points only
@@ -44,8 +44,7 @@ template <int LOG2N>
 static hipError_t launch_batch_n(bool inverse, const BatchArgs &a, BatchVariant v, hipStream_t stream) {
     if constexpr (LOG2N <= 5) {
         // contiguous batches of up to 32 points: coalesced blocks (k_batch.hpp: fft_small_kernel)
-        if (!a.done && a.batch > 0 && a.in_dist == (1u << LOG2N) && a.out_dist == (1u << LOG2N) &&
-            a.batch <= (~0ull >> 5))
+        if (a.batch > 0 && a.in_dist == (1u << LOG2N) && a.out_dist == (1u << LOG2N) && a.batch <= (~0ull >> 5))
             return launch_small<LOG2N>(inverse, v, a, stream);
     }
     constexpr int PER = batch_per_wg(LOG2N);

@@ -78,7 +78,7 @@ __device__ __forceinline__ float from_pair_lane(float x) {
 // instead of through LDS.
 template <int LOG2N, bool INV, bool NT, bool XLANE = false>
 __global__ __launch_bounds__(256) void fft_small_kernel(const cf *__restrict__ in, cf *__restrict__ out,
-                                                         uint64_t total) {
+                                                         uint64_t total, uint32_t *done, uint32_t seq) {
     constexpr int N = 1 << LOG2N;
     static_assert(LOG2N <= 5, "up to 32 points: one register DFT per 16 / n values, or two threads per 32");
     __shared__ cf lds[LOG2N > 0 ? lds_span(kSmallBlock) : 1];
@@ -151,6 +151,12 @@ __global__ __launch_bounds__(256) void fft_small_kernel(const cf *__restrict__ i
             constexpr uint32_t c = decltype(r)::value * 256u;
             if (t + c < count) io.store(t, c, v[r]);
         });
+    }
+    if (done) {
+        // as in fft_batch_kernel: every thread's stores reach the system before the flag
+        __threadfence_system();
+        __syncthreads();
+        if (t == 0) __hip_atomic_store(done, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 

@@ -59,8 +59,16 @@ constexpr int kMaxDevices = 16;
 constexpr int kDefaultSlots = 4;
 constexpr size_t kWordBytes = 256;  // the direct buffer's completion word, on its own cache line
 // how long a direct call polls its completion word before it waits on the stream instead (the
-// GPU may be busy with other work); the result is the same either way
+// GPU may be busy with other work); the result is the same either way. ONEFFT_POLL_US overrides it
+// (0: wait on the stream at once), so tests can take the fallback.
 constexpr std::chrono::microseconds kPollFor{200};
+std::chrono::microseconds poll_for() {
+    if (const char *env = std::getenv("ONEFFT_POLL_US")) {
+        const long v = std::atol(env);
+        if (v >= 0 && v <= 10000000) return std::chrono::microseconds(v);
+    }
+    return kPollFor;
+}
 
 // what a call needs besides the stream
 enum Need : unsigned { kNeedBuf = 1, kNeedRing = 2, kNeedDirect = 4 };
@@ -338,9 +346,9 @@ bool stage_out(Slot &s, void *dst, const void *d_src, size_t bytes) {
     return true;
 }
 
-// Polls the completion word until it holds `seq` (true) or kPollFor has passed (false).
+// Polls the completion word until it holds `seq` (true) or poll_for() has passed (false).
 bool poll_word(const uint32_t *word, uint32_t seq) {
-    const auto until = std::chrono::steady_clock::now() + kPollFor;
+    const auto until = std::chrono::steady_clock::now() + poll_for();
     for (;;) {
         for (int i = 0; i < 64; i++) {
             if (__atomic_load_n(word, __ATOMIC_ACQUIRE) == seq) return true;

@@ -6,6 +6,7 @@
   device buffers while every library allocation is made to fail (ONEFFT_FAIL_ALLOC=0), and every
   result is bit-identical to the device path.
 * A call with device-memory arguments orders after work queued on the default stream.
+* Calls of one LDS pass return on the kernel's completion word, or on the stream when it is late.
 * Allocation failure (ONEFFT_FAIL_ALLOC=k, the k-th library allocation fails, swept over every k):
   onefft_plan_create returns NULL, onefft_forward and onefft_cache_prepare return -1 with `out`
   untouched, onefft_ragged_plan_create and graph creation return NULL, a host call whose staging
@@ -70,6 +71,41 @@ def device_reference(p, x, n, inverse):
     p.batch_ptr(d_in.data_ptr(), d_out.data_ptr(), 1, n, n, inverse, torch.cuda.current_stream().cuda_stream)
     torch.cuda.synchronize()
     return d_out.cpu().numpy()
+
+
+@pytest.mark.parametrize("L", [0, 5, 10, 14])
+def test_completion_word_and_its_fallback_bitwise(monkeypatch, L):
+    """Calls of one LDS pass on host (pageable, pinned) and device buffers return once the kernel's
+    completion word arrives; with ONEFFT_POLL_US=0 they wait on the stream instead (the fallback
+    when the word is late). Both give the device path's bits, many calls in a row (the word's
+    sequence number advances), and the output lands only in `out`."""
+    n = 1 << L
+    p = F.Plan(n)
+    for inverse in (False, True):
+        x = O.uniform(40 + L, n)
+        want = device_reference(p, x, n, inverse)
+        call = p.inverse_ptr if inverse else p.forward_ptr
+        for poll in (None, "0"):
+            if poll is None:
+                monkeypatch.delenv("ONEFFT_POLL_US", raising=False)
+            else:
+                monkeypatch.setenv("ONEFFT_POLL_US", poll)
+            for rep in range(3):
+                g_in, g_out = Guarded(2 * n, 1), Guarded(2 * n, 3)
+                g_in.data[:] = x
+                call(g_in.ptr, g_out.ptr)
+                assert np.array_equal(g_out.data, want), (L, inverse, poll, rep, "pageable")
+                assert g_in.intact() and g_out.intact() and np.array_equal(g_in.data, x)
+                t_in = torch.from_numpy(x).pin_memory()
+                t_out = torch.full((2 * n,), float(CANARY)).pin_memory()
+                call(t_in.data_ptr(), t_out.data_ptr())
+                assert np.array_equal(t_out.numpy(), want), (L, inverse, poll, rep, "pinned")
+                d_in = torch.from_numpy(x).cuda()
+                d_out = torch.empty_like(d_in)
+                call(d_in.data_ptr(), d_out.data_ptr())
+                assert np.array_equal(d_out.cpu().numpy(), want), (L, inverse, poll, rep, "device")
+    monkeypatch.delenv("ONEFFT_POLL_US", raising=False)
+    p.close()
 
 
 def test_prepare_then_fresh_threads_allocate_nothing():

@@ -605,7 +605,11 @@ def access_patterns(torch):
     names = {0: "input-pass reads (16 columns x 1024 rows, 1 MiB apart), contiguous stores",
              1: "input-pass reads, direct row stores (a wave writes 16 rows x 4 values)",
              2: "input-pass reads, row stores through LDS (a wave writes 64 values of one row)",
-             3: "last in-place pass (32 columns x 256 rows, 4 MiB apart), in place"}
+             3: "last in-place pass (32 columns x 256 rows, 4 MiB apart), in place",
+             4: "the same in-place tiles with rows 16 KiB apart (the middle pass's stride)",
+             5: "the same in-place tiles with rows 4 MiB + 512 B apart (15874 tiles)"}
+    moved = {p: 2 * (8 << 27) for p in names}
+    moved[5] = 2 * 8 * 15874 * 256 * 32
 
     def run(p):
         if lib.onefft_access_pattern(p, a.data_ptr(), b.data_ptr(), stream.cuda_stream) != 0:
@@ -626,7 +630,7 @@ def access_patterns(torch):
     for p, ts in times.items():
         print(json.dumps({"pattern": p, "what": names[p], "min_us": round(min(ts) * 1e3, 1),
                           "median_us": round(statistics.median(ts) * 1e3, 1),
-                          "tb_s_at_min": round(2 * (8 << 27) / (min(ts) * 1e-3) / 1e12, 2),
+                          "tb_s_at_min": round(moved[p] / (min(ts) * 1e-3) / 1e12, 2),
                           "row_stores_equal": same if p in (1, 2) else None}), flush=True)
 
 

@@ -52,7 +52,11 @@ extern "C" __attribute__((visibility("default"))) int onefft_stream_copy(const v
 //   2  the same reads and rows, stored through LDS (real parts, then imaginary parts) so that a
 //      wave writes 64 consecutive values of one row;
 //   3  the last in-place pass (32 adjacent columns x 256 rows, rows 2^19 values apart, 512 threads
-//      x 16 values), read and written back in place.
+//      x 16 values), read and written back in place;
+//   4  the same tiles with rows 2^11 values (16 KiB) apart, the middle pass's stride;
+//   5  the same tiles with rows 2^19 + 64 values apart (4 MiB + 512 bytes: as many rows and pages
+//      per tile as pattern 3, the stride no power of two), over the 15874 tiles whose rows fit the
+//      buffer.
 namespace onefft {
 
 constexpr uint32_t kPatRows = 1024, kPatCols = 1u << 17;  // the input view: 1024 rows of 2^17
@@ -92,8 +96,8 @@ __global__ __launch_bounds__(1024) void column_pattern_kernel(const cf *__restri
     }
 }
 
+template <uint64_t kStride>
 __global__ __launch_bounds__(512) void inplace_pattern_kernel(cf *__restrict__ buf, float add) {
-    constexpr uint64_t kStride = 1u << 19;
     const uint32_t tile = blockIdx.x, t = threadIdx.x;
     const uint64_t col = (uint64_t)tile * 32 + (t & 31);
     const uint32_t r0 = t >> 5;
@@ -105,10 +109,26 @@ __global__ __launch_bounds__(512) void inplace_pattern_kernel(cf *__restrict__ b
     });
 }
 
+// pattern 4: tiles of 32 columns x 256 rows 2^11 values apart, the bands of 256 rows laid end to
+// end (as a middle pass's tiles lie in a 2^27-point transform split 2^8 * 2^8 * 2^11)
+__global__ __launch_bounds__(512) void inplace_pattern_band_kernel(cf *__restrict__ buf, float add) {
+    constexpr uint64_t kStride = 1u << 11;
+    const uint32_t tile = blockIdx.x, t = threadIdx.x;
+    const uint64_t band = tile / 64, col = (uint64_t)(tile % 64) * 32 + (t & 31);
+    const uint32_t r0 = t >> 5;
+    cf *b = buf + band * 256 * kStride;
+    cf v[16];
+    static_for<0, 16>([&](auto k) { v[k] = b[(uint64_t)(r0 + 16 * decltype(k)::value) * kStride + col]; });
+    static_for<0, 16>([&](auto k) {
+        v[k].r += add;
+        b[(uint64_t)(r0 + 16 * decltype(k)::value) * kStride + col] = v[k];
+    });
+}
+
 }  // namespace onefft
 
-// Runs access pattern `pattern` (0..3, above) once: `a` the input, `b` the output (pattern 3
-// works in place on `b` and leaves its values unchanged). Both hold 2^27 complex values, device
+// Runs access pattern `pattern` (0..5, above) once: `a` the input, `b` the output (patterns 3..5
+// work in place on `b` and leave its values unchanged). Both hold 2^27 complex values, device
 // memory, 8-byte aligned. Returns 0, or -1 for an unknown pattern or a failed launch.
 extern "C" __attribute__((visibility("default"))) int onefft_access_pattern(int pattern, const void *a, void *b,
                                                                            hipStream_t stream) {
@@ -119,7 +139,12 @@ extern "C" __attribute__((visibility("default"))) int onefft_access_pattern(int 
     case 0: hipLaunchKernelGGL(column_pattern_kernel<0>, tiles, dim3(1024), 0, stream, (const cf *)a, (cf *)b); break;
     case 1: hipLaunchKernelGGL(column_pattern_kernel<1>, tiles, dim3(1024), 0, stream, (const cf *)a, (cf *)b); break;
     case 2: hipLaunchKernelGGL(column_pattern_kernel<2>, tiles, dim3(1024), 0, stream, (const cf *)a, (cf *)b); break;
-    case 3: hipLaunchKernelGGL(inplace_pattern_kernel, dim3((1u << 19) / 32), dim3(512), 0, stream, (cf *)b, 0.0f); break;
+    case 3: hipLaunchKernelGGL(inplace_pattern_kernel<1u << 19>, dim3((1u << 19) / 32), dim3(512), 0, stream, (cf *)b, 0.0f); break;
+    // 2^27 values: 2^27 / (256 rows x 32 columns) tiles; with rows 2^11 apart, tile k covers column
+    // block k mod 64 of the 256-row band k / 64
+    case 4: hipLaunchKernelGGL(inplace_pattern_band_kernel, dim3((1u << 27) / (256 * 32)), dim3(512), 0, stream, (cf *)b, 0.0f); break;
+    // 255 (2^19 + 64) + 15874 x 32 <= 2^27
+    case 5: hipLaunchKernelGGL(inplace_pattern_kernel<(1u << 19) + 64>, dim3(15874), dim3(512), 0, stream, (cf *)b, 0.0f); break;
     default: return -1;
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;

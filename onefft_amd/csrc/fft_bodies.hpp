@@ -195,7 +195,7 @@ __host__ __device__ constexpr uint32_t pass_span(int log2nf, int cols, int layou
     return need + (u >= r ? u - r : 3 * u - r);
 }
 
-template <int LOG2NF, int C, bool FIRST, bool SWAP_IN, bool SWAP_OUT, int LAYOUT = kPadded>
+template <int LOG2NF, int C, bool FIRST, bool SWAP_IN, bool SWAP_OUT, int LAYOUT = kPadded, bool TWCOL = false>
 __device__ __forceinline__ void pass_body(const PassArgs &a, const cf *__restrict__ src, cf *__restrict__ dst,
                                           uint32_t tile, cf *lds, uint32_t t) {
     static_assert(LAYOUT == kPadded || LAYOUT == kHalves, "pass kernels use the padded or the halves layout");
@@ -263,9 +263,22 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, const cf *__restric
             if constexpr (!FIRST) {
                 // element e = j + T q + STRIDE r gets w_n^((e low) << tw_shift), formed in FP64
                 auto W = [&](uint32_t x) { return two_level(a.tw_hi, a.tw_lo, a.lo_bits, x << a.tw_shift); };
-                cd wq = W(j * low);
-                const cd step_q = W(T * low);
-                const cd step_r = W(STRIDE * low);
+                cd wq, step_q, step_r;
+                if constexpr (TWCOL) {
+                    // low = low0 + c, low0 the tile's first column (C divides M_f): w^(x low) =
+                    // w^(x low0) w^(x c). The first factor has T distinct values per workgroup
+                    // (nearly broadcast loads), the second comes from the pass's column table, read
+                    // lane by lane in order; three independent lookups of x low per lane would
+                    // gather 64 scattered entries from each table level
+                    const uint32_t low0 = (tile * C) & ((1u << a.log2m) - 1);
+                    wq = mul64(W(j * low0), a.tw_col[t]);
+                    step_q = mul64(W(T * low0), a.tw_col[T * C + c]);
+                    step_r = mul64(W(STRIDE * low0), a.tw_col[T * C + C + c]);
+                } else {
+                    wq = W(j * low);
+                    step_q = W(T * low);
+                    step_r = W(STRIDE * low);
+                }
                 static_for<0, BF>([&](auto q) {
                     constexpr int qq = decltype(q)::value;
                     cd w = wq;

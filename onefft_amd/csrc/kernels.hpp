@@ -92,6 +92,10 @@ struct PassArgs {
     const cd *tw64;       // FP64 bases of the sub-transform's stages (fft_common.hpp: stage_has_tw64)
     const cd *tw_lo;      // w_n^i, i < 2^lo_bits (FP64)
     const cd *tw_hi;      // w_n^(i 2^lo_bits), i < n / 2^lo_bits (FP64)
+    // in-place passes whose C columns per workgroup divide M_f and whose lookups would miss the
+    // L1 (runtime.hip): w_n^(x << tw_shift) for x = j c at thread t = j C + c (T C entries), then
+    // x = T c and x = (N_f / R_0) c (C each); null otherwise
+    const cd *tw_col;
     uint64_t in_dist;     // complex values between transforms of a batch
     uint64_t out_dist;
     const uint64_t *offsets;  // if set, transform b starts at offsets[b] complex values in both

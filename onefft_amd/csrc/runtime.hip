@@ -9,6 +9,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -274,7 +275,9 @@ onefft_plan *onefft::plan_build(size_t n, bool staging) {
         return off;
     };
     size_t off_single = 0, off_single64 = 0, off_stage64[2][kMaxPasses] = {};
-    size_t off_lo = 0, off_hi = 0;
+    size_t off_lo = 0, off_hi = 0, off_col[2][kMaxPasses];
+    for (auto &per : off_col)
+        for (size_t &o : per) o = SIZE_MAX;  // no column table
     if (log2n <= (unsigned)kMaxSinglePassLog2) {
         const int lp = batch_log2p((int)log2n);
         const uint32_t entries = stage_tw_size((int)log2n, lp);
@@ -313,6 +316,22 @@ onefft_plan *onefft::plan_build(size_t n, bool staging) {
                 a.log2n = log2n;
                 a.log2m = log2m;
                 a.tw_shift = log2n - log2m - (unsigned)lf;
+                if (log2m > 0 && (uint64_t)cols[f - 1] <= (uint64_t(1) << log2m) && lo_bits > a.tw_shift + 10) {
+                    // the in-place pass's column table (kernels.hpp: PassArgs::tw_col), where the
+                    // per-lane lookups would sweep more than 2^10 entries (16 KiB) of the low level,
+                    // w_n^(x << tw_shift) for x < 2^(lo_bits - tw_shift), and so miss the L1; below
+                    // that the lookups hit the L1 and the table's extra products cost more
+                    const uint32_t C = (uint32_t)cols[f - 1], T = 1u << (lf - lp);
+                    const uint32_t stride = 1u << (lf - stage_log2r(lf, lp, 0));
+                    off_col[r][i] = reserve((T * C + 2 * C) * sizeof(cd));
+                    cd *col = reinterpret_cast<cd *>(image.data() + off_col[r][i]);
+                    auto root = [&](uint64_t x, cd *w) { root64((x << a.tw_shift) & (n - 1), log2n, &w->r, &w->i); };
+                    for (uint32_t t = 0; t < T * C; t++) root((uint64_t)(t / C) * (t % C), &col[t]);
+                    for (uint32_t c = 0; c < C; c++) {
+                        root((uint64_t)T * c, &col[T * C + c]);
+                        root((uint64_t)stride * c, &col[T * C + C + c]);
+                    }
+                }
                 a.columns = (uint32_t)(n >> lf);
                 // ONEFFT_PASS_STRIPS: strips of the input pass's tile order (for measurement)
                 a.strips = 1;
@@ -354,6 +373,7 @@ onefft_plan *onefft::plan_build(size_t n, bool staging) {
                 a.tw64 = reinterpret_cast<const cd *>(dev(off_stage64[r][i]));
                 a.tw_lo = reinterpret_cast<const cd *>(dev(off_lo));
                 a.tw_hi = reinterpret_cast<const cd *>(dev(off_hi));
+                a.tw_col = off_col[r][i] == SIZE_MAX ? nullptr : reinterpret_cast<const cd *>(dev(off_col[r][i]));
             }
         }
     }

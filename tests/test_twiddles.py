@@ -59,7 +59,7 @@ def tw(tmp_path_factory):
     L.tw_two_level_lo_bits.restype = ctypes.c_uint
     L.tw_two_level.argtypes = [ctypes.c_uint, ctypes.c_void_p, ctypes.c_void_p]
     L.tw_count_single.argtypes = [ctypes.c_int, ctypes.c_int, P64]
-    L.tw_count_pass.argtypes = [ctypes.c_uint, ctypes.c_int, ctypes.c_uint, ctypes.c_uint32, P64, P64]
+    L.tw_count_pass.argtypes = [ctypes.c_uint, ctypes.c_int, ctypes.c_uint, ctypes.c_uint32, ctypes.c_uint32, P64, P64]
     L.tw_count_ragged_pass.argtypes = [ctypes.c_uint, ctypes.c_uint32, P64]
     return L
 
@@ -190,18 +190,19 @@ def test_single_pass_formed_twiddles(tw, log2n):
     print(f"single 2^{log2n}: (checked, zero residues, one-ulp, other, residue) per FP64-base level 1..3: {rows}")
 
 
-def pass_shapes(split):
-    """(lf, log2m) of each pass in execution order for a split log2 N_1 .. log2 N_p (k_pass.hpp)."""
+def pass_shapes(split, cols):
+    """(lf, log2m, columns per workgroup) of each pass in execution order for a split
+    log2 N_1 .. log2 N_p and its columns, listed in the same order (k_pass.hpp)."""
     p = len(split)
     out = []
     for i in range(p):
         f = p - i
-        out.append((split[f - 1], sum(split[f:])))
+        out.append((split[f - 1], sum(split[f:]), cols[f - 1]))
     return out
 
 
 def tuned_splits():
-    """The multi-pass splits the committed tuning table picks, per size and regime."""
+    """The multi-pass splits and columns the committed tuning table picks, per size and regime."""
     path = os.path.join(ROOT, "onefft_amd", "tune_gfx950.tsv")
     out = []
     with open(path) as f:
@@ -210,20 +211,22 @@ def tuned_splits():
                 continue
             log2n, kind, params = line.split("\t")[:3]
             if kind == "multi":
-                split = [int(v) for v in params.split(";")[0][len("split="):].split(",")]
-                out.append((int(log2n), split))
+                fields = params.split(";")
+                split = [int(v) for v in fields[0][len("split="):].split(",")]
+                cols = [int(v) for v in fields[1][len("cols="):].split(",")]
+                out.append((int(log2n), split, cols))
     return out
 
 
-@pytest.mark.parametrize("log2n,split", tuned_splits())
-def test_pass_formed_twiddles(tw, log2n, split):
+@pytest.mark.parametrize("log2n,split,cols", tuned_splits())
+def test_pass_formed_twiddles(tw, log2n, split, cols):
     """Every pass of a tuned split: pass twiddles for every low index (every 2^k-th above 2^24
     points, to bound the time) and the sub-transform's stage recurrences."""
     step = 1 if log2n <= 24 else 1 << (log2n - 24)
     rows = []
-    for lf, log2m in pass_shapes(split):
+    for lf, log2m, c in pass_shapes(split, cols):
         op, os_ = (U64 * 5)(), (U64 * 5)()
-        tw.tw_count_pass(log2n, lf, log2m, step, op, os_)
+        tw.tw_count_pass(log2n, lf, log2m, c, step, op, os_)
         cp, cs = unpack(op), unpack(os_)
         if log2m:
             check_counts(f"pass 2^{log2n} {split} lf={lf}", *cp)

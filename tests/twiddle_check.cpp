@@ -10,7 +10,9 @@
 //   * stage recurrences: w^(r k) = base * base * ... (r - 1 products), base = w_{Ns R}^k
 //     (fft_bodies.hpp single_finish / pass_body stages s > 0, k_ragged.hip column_fft);
 //   * pass twiddles: w_n^((j + T q + STRIDE r) low << shift) = W(j low) W(T low)^q W(STRIDE low)^r
-//     with W the two-level product (fft_bodies.hpp pass_body stage 0, k_ragged.hip first_stage).
+//     with W the two-level product (fft_bodies.hpp pass_body stage 0, k_ragged.hip first_stage),
+//     or, in passes with a column table, with each base W(x low) formed as W(x low0) times the
+//     table's entry w^(x c) (low = low0 + c, c < C columns per workgroup).
 // Each formed value is compared with root32 of the same exponent, i.e. with the reference's
 // onefft_root (which the test pins root32 against). Two kinds of difference are expected: a
 // component that onefft_root holds as exactly 0 (its octant folding makes quarter turns exact)
@@ -153,10 +155,15 @@ void tw_count_single(int log2n, int level, uint64_t *out) {
 }
 
 // One pass of a multi-pass transform of 2^log2n points: sub-transforms of 2^lf points (16 per
-// thread), low index extent 2^log2m (0 for the input pass, which has no pass twiddles). Counts
-// the pass twiddles for every low index that is a multiple of `low_step` (1 for all), and the
-// stage recurrences of the sub-transform (every stage after the first, level 3).
-void tw_count_pass(unsigned log2n, int lf, unsigned log2m, uint32_t low_step, uint64_t *out_pass,
+// thread), `cols` columns per workgroup, low index extent 2^log2m (0 for the input pass, which
+// has no pass twiddles). Counts the pass twiddles for one low index in every `low_step` (all for
+// 1; otherwise base + (base / low_step) mod low_step for every multiple base, so that the column
+// within the tile still varies), and the stage recurrences of the sub-transform (every stage
+// after the first, level 3). Where the plan gives the pass a column table (cols <= 2^log2m and
+// lo_bits > shift + 10, runtime.hip) the bases come as the kernels form them from it
+// (fft_bodies.hpp pass_body): w^(x low) = W(x low0) w^(x c) for low = low0 + c, c < cols, w^(x c)
+// an exact root64; otherwise as three two-level lookups.
+void tw_count_pass(unsigned log2n, int lf, unsigned log2m, uint32_t cols, uint32_t low_step, uint64_t *out_pass,
                    uint64_t *out_stage) {
     Count stage;
     const int lp = lf < 4 ? lf : 4;  // kernels.hpp: pass_log2p
@@ -172,10 +179,20 @@ void tw_count_pass(unsigned log2n, int lf, unsigned log2m, uint32_t low_step, ui
         const uint32_t NF = 1u << lf, P = 1u << lp, T = NF / P;
         const uint32_t R = 1u << onefft::stage_log2r(lf, lp, 0), BF = P / R, STRIDE = NF / R;
         auto W = [&](uint32_t x) { return onefft::two_level(hi, lo, lo_bits, x << shift); };
-        for (uint64_t low = 0; low < (uint64_t(1) << log2m); low += low_step) {
+        auto exact = [&](uint64_t x) {
+            cd w;
+            onefft::root64((x << shift) & ((uint64_t(1) << log2n) - 1), log2n, &w.r, &w.i);
+            return w;
+        };
+        const bool table = cols <= (1u << log2m) && lo_bits > shift + 10;
+        for (uint64_t base = 0; base < (uint64_t(1) << log2m); base += low_step) {
+            const uint64_t low = base + (base / low_step) % low_step;
+            const uint32_t c = table ? (uint32_t)low & (cols - 1) : 0, low0 = (uint32_t)low - c;
             for (uint32_t j = 0; j < T; j++) {
-                cd wq = W(j * (uint32_t)low);
-                const cd step_q = W(T * (uint32_t)low), step_r = W(STRIDE * (uint32_t)low);
+                cd wq = table ? onefft::mul64(W(j * low0), exact((uint64_t)j * c)) : W(j * (uint32_t)low);
+                const cd step_q = table ? onefft::mul64(W(T * low0), exact((uint64_t)T * c)) : W(T * (uint32_t)low);
+                const cd step_r =
+                    table ? onefft::mul64(W(STRIDE * low0), exact((uint64_t)STRIDE * c)) : W(STRIDE * (uint32_t)low);
                 for (uint32_t q = 0; q < BF; q++) {
                     cd w = wq;
                     for (uint32_t r = 0; r < R; r++) {

@@ -195,7 +195,7 @@ __host__ __device__ constexpr uint32_t pass_span(int log2nf, int cols, int layou
     return need + (u >= r ? u - r : 3 * u - r);
 }
 
-template <int LOG2NF, int C, bool FIRST, bool SWAP_IN, bool SWAP_OUT, int LAYOUT = kPadded, bool TWCOL = false>
+template <int LOG2NF, int C, bool FIRST, bool SWAP_IN, bool SWAP_OUT, int LAYOUT = kPadded, bool COL_STEPS = false>
 __device__ __forceinline__ void pass_body(const PassArgs &a, const cf *__restrict__ src, cf *__restrict__ dst,
                                           uint32_t tile, cf *lds, uint32_t t) {
     static_assert(LAYOUT == kPadded || LAYOUT == kHalves, "pass kernels use the padded or the halves layout");
@@ -261,21 +261,22 @@ __device__ __forceinline__ void pass_body(const PassArgs &a, const cf *__restric
                 });
             });
             if constexpr (!FIRST) {
-                // element e = j + T q + STRIDE r gets w_n^((e low) << tw_shift), formed in FP64
+                // element e = j + T q + STRIDE r gets w_n^((e low) << tw_shift), formed in FP64 from
+                // the bases w^(j low), w^(T low) and w^(STRIDE low). With low = low0 + (c mod M_f),
+                // low0 the tile's first column mod M_f (uniform in the workgroup), w^(x low) =
+                // w^(x low0) w^(x (c mod M_f)): the first factor takes at most T distinct values per
+                // workgroup (nearly broadcast loads), the second comes from the pass's column table,
+                // read lane by lane in order. Looked up directly, w^(j low) gathers 64 scattered
+                // entries per wave. The step bases are looked up directly unless COL_STEPS (passes
+                // whose lookups of x low sweep more of the low table than the L1 holds, runtime.hip)
                 auto W = [&](uint32_t x) { return two_level(a.tw_hi, a.tw_lo, a.lo_bits, x << a.tw_shift); };
-                cd wq, step_q, step_r;
-                if constexpr (TWCOL) {
-                    // low = low0 + c, low0 the tile's first column (C divides M_f): w^(x low) =
-                    // w^(x low0) w^(x c). The first factor has T distinct values per workgroup
-                    // (nearly broadcast loads), the second comes from the pass's column table, read
-                    // lane by lane in order; three independent lookups of x low per lane would
-                    // gather 64 scattered entries from each table level
-                    const uint32_t low0 = (tile * C) & ((1u << a.log2m) - 1);
-                    wq = mul64(W(j * low0), a.tw_col[t]);
+                const uint32_t low0 = (tile * C) & ((1u << a.log2m) - 1);
+                cd wq = mul64(W(j * low0), a.tw_col[t]);
+                cd step_q, step_r;
+                if constexpr (COL_STEPS) {
                     step_q = mul64(W(T * low0), a.tw_col[T * C + c]);
                     step_r = mul64(W(STRIDE * low0), a.tw_col[T * C + C + c]);
                 } else {
-                    wq = W(j * low);
                     step_q = W(T * low);
                     step_r = W(STRIDE * low);
                 }

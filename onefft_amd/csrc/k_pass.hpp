@@ -16,9 +16,10 @@
 //
 // The pass twiddles w_n^x span all n exponents. They are formed in FP64 from two small FP64
 // tables (w_n^x = w_n^(x_hi 2^b) w_n^(x_lo)) and a short FP64 recurrence per thread, then rounded
-// once to FP32, without reading an n-entry table from HBM. In passes whose lookups would sweep
-// more of the low table than the L1 holds, the plan adds a column table and the recurrence's
-// bases are formed per tile from it (PassArgs::tw_col, instantiations with TWCOL). The values equal the reference's
+// once to FP32, without reading an n-entry table from HBM. The in-place passes form their
+// recurrence's bases per tile with a small column table of the plan (PassArgs::tw_col; the step
+// bases too in passes whose lookups would sweep more of the low table than the L1 holds,
+// instantiations with COL_STEPS). The values equal the reference's
 // FP64-computed, once-rounded twiddles (src/onefft.c:112-141) except where a component that the
 // reference holds as exactly 0 (a quarter turn) comes out as a residue of a few 1e-16, or where
 // the exact value lies within the FP64 formation error of an FP32 rounding boundary (then one
@@ -29,7 +30,7 @@
 
 namespace onefft {
 
-template <int LOG2NF, int C, bool FIRST, bool SWAP_IN, bool SWAP_OUT, int LAYOUT, bool TWCOL>
+template <int LOG2NF, int C, bool FIRST, bool SWAP_IN, bool SWAP_OUT, int LAYOUT, bool COL_STEPS>
 __global__ __launch_bounds__(pass_wg(LOG2NF, C)) void fft_pass_kernel(PassArgs a) {
     constexpr bool USES_LDS = stage_count(LOG2NF, pass_log2p(LOG2NF)) > 1;
     // C columns of pass_span values: complex values, or floats with the halves layout
@@ -42,7 +43,7 @@ __global__ __launch_bounds__(pass_wg(LOG2NF, C)) void fft_pass_kernel(PassArgs a
     const uint32_t tile = w % tiles, which = w / tiles;
     const uint64_t in_at = a.offsets ? a.offsets[which] : (uint64_t)which * a.in_dist;
     const uint64_t out_at = a.offsets ? a.offsets[which] : (uint64_t)which * a.out_dist;
-    pass_body<LOG2NF, C, FIRST, SWAP_IN, SWAP_OUT, LAYOUT, TWCOL>(a, a.in + in_at, a.out + out_at, tile, lds, threadIdx.x);
+    pass_body<LOG2NF, C, FIRST, SWAP_IN, SWAP_OUT, LAYOUT, COL_STEPS>(a, a.in + in_at, a.out + out_at, tile, lds, threadIdx.x);
 }
 
 template <int LOG2NF, int C, int LAYOUT>
@@ -52,15 +53,17 @@ inline hipError_t launch_pass_nc(bool first, bool swap_in, bool swap_out, const 
         return hipErrorInvalidValue;
     } else {
         if (a.columns % C || batch == 0 || batch > 65535) return hipErrorInvalidValue;
+        if (!first && !a.tw_col) return hipErrorInvalidValue;
         dim3 grid(a.columns / C, (uint32_t)batch), block(pass_wg(LOG2NF, C));
-        // in-place passes with a column table (PassArgs::tw_col) run their own instantiation: a
-        // run-time branch between the two twiddle formations costs registers and time in both
+        // in-place passes that take their step bases from the column table (PassArgs::col_steps)
+        // run their own instantiation: a run-time branch between the two formations cost
+        // registers and time in both
         if (first) {
             if (swap_in)
                 hipLaunchKernelGGL((fft_pass_kernel<LOG2NF, C, true, true, false, LAYOUT, false>), grid, block, 0, stream, a);
             else
                 hipLaunchKernelGGL((fft_pass_kernel<LOG2NF, C, true, false, false, LAYOUT, false>), grid, block, 0, stream, a);
-        } else if (a.tw_col) {
+        } else if (a.col_steps) {
             if (swap_out)
                 hipLaunchKernelGGL((fft_pass_kernel<LOG2NF, C, false, false, true, LAYOUT, true>), grid, block, 0, stream, a);
             else

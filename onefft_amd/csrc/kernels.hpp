@@ -92,9 +92,9 @@ struct PassArgs {
     const cd *tw64;       // FP64 bases of the sub-transform's stages (fft_common.hpp: stage_has_tw64)
     const cd *tw_lo;      // w_n^i, i < 2^lo_bits (FP64)
     const cd *tw_hi;      // w_n^(i 2^lo_bits), i < n / 2^lo_bits (FP64)
-    // in-place passes whose C columns per workgroup divide M_f and whose lookups would miss the
-    // L1 (runtime.hip): w_n^(x << tw_shift) for x = j c at thread t = j C + c (T C entries), then
-    // x = T c and x = (N_f / R_0) c (C each); null otherwise
+    // in-place passes, the column table: w_n^(x << tw_shift) for x = j (c mod M_f) at thread
+    // t = j C + c (T C entries), then x = T (c mod M_f) and x = (N_f / R_0) (c mod M_f) (C each);
+    // null for the input pass
     const cd *tw_col;
     uint64_t in_dist;     // complex values between transforms of a batch
     uint64_t out_dist;
@@ -110,6 +110,7 @@ struct PassArgs {
     uint32_t dig_shift[kMaxPasses];  // log2 of each digit's weight in the row index
     uint32_t columns;     // n / N_f
     uint32_t strips;      // tile order: XCD-contiguous ranges run as this many strips side by side
+    uint32_t col_steps;   // in-place passes: the recurrence's step bases from tw_col as well
 };
 
 // One pass of a multi-pass transform: first = reads the input (out of place), else in place on

@@ -316,21 +316,24 @@ onefft_plan *onefft::plan_build(size_t n, bool staging) {
                 a.log2n = log2n;
                 a.log2m = log2m;
                 a.tw_shift = log2n - log2m - (unsigned)lf;
-                if (log2m > 0 && (uint64_t)cols[f - 1] <= (uint64_t(1) << log2m) && lo_bits > a.tw_shift + 10) {
-                    // the in-place pass's column table (kernels.hpp: PassArgs::tw_col), where the
-                    // per-lane lookups would sweep more than 2^10 entries (16 KiB) of the low level,
-                    // w_n^(x << tw_shift) for x < 2^(lo_bits - tw_shift), and so miss the L1; below
-                    // that the lookups hit the L1 and the table's extra products cost more
+                if (log2m > 0) {
+                    // the in-place pass's column table (kernels.hpp: PassArgs::tw_col); its step
+                    // bases come from it too where the lookups of x low would sweep more than
+                    // 2^10 entries (16 KiB) of the low level, w_n^(x << tw_shift) for
+                    // x < 2^(lo_bits - tw_shift), and so miss the L1; below that they hit the L1
+                    // and the table's two more products cost more than they save
                     const uint32_t C = (uint32_t)cols[f - 1], T = 1u << (lf - lp);
                     const uint32_t stride = 1u << (lf - stage_log2r(lf, lp, 0));
+                    const uint32_t mask = (1u << log2m) - 1;  // c mod M_f: below C when C > M_f
                     off_col[r][i] = reserve((T * C + 2 * C) * sizeof(cd));
                     cd *col = reinterpret_cast<cd *>(image.data() + off_col[r][i]);
                     auto root = [&](uint64_t x, cd *w) { root64((x << a.tw_shift) & (n - 1), log2n, &w->r, &w->i); };
-                    for (uint32_t t = 0; t < T * C; t++) root((uint64_t)(t / C) * (t % C), &col[t]);
+                    for (uint32_t t = 0; t < T * C; t++) root((uint64_t)(t / C) * ((t % C) & mask), &col[t]);
                     for (uint32_t c = 0; c < C; c++) {
-                        root((uint64_t)T * c, &col[T * C + c]);
-                        root((uint64_t)stride * c, &col[T * C + C + c]);
+                        root((uint64_t)T * (c & mask), &col[T * C + c]);
+                        root((uint64_t)stride * (c & mask), &col[T * C + C + c]);
                     }
+                    a.col_steps = lo_bits > a.tw_shift + 10 ? 1 : 0;
                 }
                 a.columns = (uint32_t)(n >> lf);
                 // ONEFFT_PASS_STRIPS: strips of the input pass's tile order (for measurement)

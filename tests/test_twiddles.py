@@ -218,7 +218,11 @@ def tuned_splits():
     return out
 
 
-@pytest.mark.parametrize("log2n,split,cols", tuned_splits())
+# besides the tuned splits: in-place passes with more columns per tile than their low extent
+WIDE_TILE_SPLITS = [(16, [8, 4, 4], [64, 64, 64]), (17, [8, 5, 4], [64, 32, 64])]
+
+
+@pytest.mark.parametrize("log2n,split,cols", tuned_splits() + WIDE_TILE_SPLITS)
 def test_pass_formed_twiddles(tw, log2n, split, cols):
     """Every pass of a tuned split: pass twiddles for every low index (every 2^k-th above 2^24
     points, to bound the time) and the sub-transform's stage recurrences."""

@@ -10,9 +10,9 @@
 //   * stage recurrences: w^(r k) = base * base * ... (r - 1 products), base = w_{Ns R}^k
 //     (fft_bodies.hpp single_finish / pass_body stages s > 0, k_ragged.hip column_fft);
 //   * pass twiddles: w_n^((j + T q + STRIDE r) low << shift) = W(j low) W(T low)^q W(STRIDE low)^r
-//     with W the two-level product (fft_bodies.hpp pass_body stage 0, k_ragged.hip first_stage),
-//     or, in passes with a column table, with each base W(x low) formed as W(x low0) times the
-//     table's entry w^(x c) (low = low0 + c, c < C columns per workgroup).
+//     with W the two-level product (k_ragged.hip first_stage); the pass kernels form the base
+//     W(j low), and in some passes the other two, as W(x low0) times the plan's column-table entry
+//     w^(x c) (low = low0 + c, c the column within the tile; fft_bodies.hpp pass_body stage 0).
 // Each formed value is compared with root32 of the same exponent, i.e. with the reference's
 // onefft_root (which the test pins root32 against). Two kinds of difference are expected: a
 // component that onefft_root holds as exactly 0 (its octant folding makes quarter turns exact)
@@ -159,10 +159,10 @@ void tw_count_single(int log2n, int level, uint64_t *out) {
 // has no pass twiddles). Counts the pass twiddles for one low index in every `low_step` (all for
 // 1; otherwise base + (base / low_step) mod low_step for every multiple base, so that the column
 // within the tile still varies), and the stage recurrences of the sub-transform (every stage
-// after the first, level 3). Where the plan gives the pass a column table (cols <= 2^log2m and
-// lo_bits > shift + 10, runtime.hip) the bases come as the kernels form them from it
-// (fft_bodies.hpp pass_body): w^(x low) = W(x low0) w^(x c) for low = low0 + c, c < cols, w^(x c)
-// an exact root64; otherwise as three two-level lookups.
+// after the first, level 3). The bases come as the kernels form them with the plan's column table
+// (runtime.hip, fft_bodies.hpp pass_body): w^(j low) = W(j low0) w^(j cm) for low = low0 + cm,
+// cm the column within the tile mod 2^log2m, w^(j cm) an exact root64; the step bases w^(T low)
+// and w^(STRIDE low) the same way where lo_bits > shift + 10, else as two-level lookups.
 void tw_count_pass(unsigned log2n, int lf, unsigned log2m, uint32_t cols, uint32_t low_step, uint64_t *out_pass,
                    uint64_t *out_stage) {
     Count stage;
@@ -184,15 +184,16 @@ void tw_count_pass(unsigned log2n, int lf, unsigned log2m, uint32_t cols, uint32
             onefft::root64((x << shift) & ((uint64_t(1) << log2n) - 1), log2n, &w.r, &w.i);
             return w;
         };
-        const bool table = cols <= (1u << log2m) && lo_bits > shift + 10;
+        const bool steps = lo_bits > shift + 10;  // runtime.hip: PassArgs::col_steps
         for (uint64_t base = 0; base < (uint64_t(1) << log2m); base += low_step) {
             const uint64_t low = base + (base / low_step) % low_step;
-            const uint32_t c = table ? (uint32_t)low & (cols - 1) : 0, low0 = (uint32_t)low - c;
+            // c mod M for the column c of the tile that holds low, and the tile's first column mod M
+            const uint32_t cm = (uint32_t)low & (cols - 1), low0 = (uint32_t)low - cm;
             for (uint32_t j = 0; j < T; j++) {
-                cd wq = table ? onefft::mul64(W(j * low0), exact((uint64_t)j * c)) : W(j * (uint32_t)low);
-                const cd step_q = table ? onefft::mul64(W(T * low0), exact((uint64_t)T * c)) : W(T * (uint32_t)low);
+                cd wq = onefft::mul64(W(j * low0), exact((uint64_t)j * cm));
+                const cd step_q = steps ? onefft::mul64(W(T * low0), exact((uint64_t)T * cm)) : W(T * (uint32_t)low);
                 const cd step_r =
-                    table ? onefft::mul64(W(STRIDE * low0), exact((uint64_t)STRIDE * c)) : W(STRIDE * (uint32_t)low);
+                    steps ? onefft::mul64(W(STRIDE * low0), exact((uint64_t)STRIDE * cm)) : W(STRIDE * (uint32_t)low);
                 for (uint32_t q = 0; q < BF; q++) {
                     cd w = wq;
                     for (uint32_t r = 0; r < R; r++) {

@@ -299,12 +299,13 @@ def test_halves_wide_tiles(monkeypatch, split, cols, batch):
     check_split(monkeypatch, split, cols, batch, 2)
 
 
-@pytest.mark.parametrize("split,cols,batch", [("8,4,4", "64,64,64", 2), ("8,5,4", "64,32,64", 1)])
-def test_columns_wider_than_low_extent(monkeypatch, split, cols, batch):
+@pytest.mark.parametrize("split,cols,batch,layout", [("8,4,4", "64,64,64", 2, 2), ("8,5,4", "64,32,64", 1, 2),
+                                                     ("7,4,4", "64,64,64", 2, 0)])
+def test_columns_wider_than_low_extent(monkeypatch, split, cols, batch, layout):
     """In-place passes with more columns per tile than their low-index extent M_f (here 64 and 32
-    columns over M_f = 16): a tile then spans several of the pass's sub-transform groups, and the
-    column table's entries are those of the column mod M_f."""
-    check_split(monkeypatch, split, cols, batch, 2)
+    columns over M_f = 16), in both LDS layouts: a tile then spans several of the pass's
+    sub-transform groups, and the column table's entries are those of the column mod M_f."""
+    check_split(monkeypatch, split, cols, batch, layout)
 
 
 def check_split(monkeypatch, split, cols, batch, layout):

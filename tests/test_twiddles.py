@@ -219,7 +219,7 @@ def tuned_splits():
 
 
 # besides the tuned splits: in-place passes with more columns per tile than their low extent
-WIDE_TILE_SPLITS = [(16, [8, 4, 4], [64, 64, 64]), (17, [8, 5, 4], [64, 32, 64])]
+WIDE_TILE_SPLITS = [(16, [8, 4, 4], [64, 64, 64]), (17, [8, 5, 4], [64, 32, 64]), (15, [7, 4, 4], [64, 64, 64])]
 
 
 @pytest.mark.parametrize("log2n,split,cols", tuned_splits() + WIDE_TILE_SPLITS)

@@ -104,7 +104,8 @@ def test_persistent_grid_bitwise(monkeypatch, grid, order):
     check_items(items, x, got, extent)
 
 
-@pytest.mark.parametrize("split,tile_run", [("20", "1"), ("19", "3"), ("18", "256"), ("21", "2")])
+@pytest.mark.parametrize("split,tile_run", [("20", "1"), ("19", "3"), ("18", "256"), ("21", "2"), ("15", "2"),
+                                             ("14", "1")])
 def test_split_and_tile_run_bitwise(monkeypatch, split, tile_run):
     """Other split sizes and tile-run lengths (ONEFFT_RAGGED_SPLIT / ONEFFT_RAGGED_TILE_RUN) change
     only which workgroup computes what: the results equal the defaults' bit for bit."""

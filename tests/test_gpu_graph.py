@@ -14,7 +14,8 @@ def stream():
     return torch.cuda.current_stream().cuda_stream
 
 
-@pytest.mark.parametrize("L,batch,inverse", [(10, 300, False), (12, 64, True), (16, 3, False), (20, 1, True)])
+@pytest.mark.parametrize("L,batch,inverse", [(10, 300, False), (12, 64, True), (16, 3, False), (20, 1, True),
+                                              (22, 2, False), (24, 1, True)])
 def test_batch_graph_replays(L, batch, inverse):
     n = 1 << L
     x = torch.from_numpy(O.normal(50 + L, n * batch)).cuda()

@@ -239,10 +239,14 @@ def test_batch(L, batch):
             assert O.rel_rms(y[seg], want[seg]) < tol_fp64(L), (b, inverse)
 
 
-@pytest.mark.parametrize("L", [4, 9, 12, 16])
-def test_batch_with_distances(L):
+@pytest.mark.parametrize("L,in_extra,out_extra", [(4, 24, 8), (9, 24, 8), (12, 24, 8), (16, 24, 8), (17, 3, 1),
+                                                  (19, 24, 1)])
+def test_batch_with_distances(L, in_extra, out_extra):
+    """Distances beyond the size, odd ones included: with an odd output distance every other
+    transform's output starts 8 but not 16 bytes aligned (the input passes of 2^17 and 2^19 store
+    16-byte pairs only to 16-byte aligned rows)."""
     n = 1 << L
-    batch, in_dist, out_dist = 6, n + 24, n + 8
+    batch, in_dist, out_dist = 6, n + in_extra, n + out_extra
     x = O.uniform(77, in_dist * batch)
     y = gpu_batch(x, n, batch, False, in_dist, out_dist)
     for b in range(batch):
